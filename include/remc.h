@@ -1,0 +1,199 @@
+/*
+ * remc.h -- C ABI of the MI355X replica-exchange Monte Carlo hot path.
+ *
+ * The reference drives its GPU through the extern "C" block of
+ * "simulation engine/src/cudaExterns.h" (lines 11-78) plus the device-mirror
+ * methods of Replica (Replica.cpp:1108-1121, 1308-1417, 1530-1576, 1601-1655).
+ * This header is the replacement boundary: plain pointers and sizes, no GPU
+ * runtime types, every call returns an int status (0 = OK, < 0 = error; the
+ * text is in remc_last_error()).  The reference aborts through cutilCheckMsg
+ * instead; nothing here calls exit().
+ *
+ * Ownership: the library owns all device memory.  Host arrays passed in are
+ * copied before the call returns.  A context is bound to one device, uses one
+ * stream and is not thread-safe (callers serialise), as the reference's
+ * one-host-thread-per-GPU model assumes (main.cpp:899-906).  Unlike the
+ * reference's per-context globals (const_boxdim, LJTexture), box size and force
+ * field constants are per context/system.
+ *
+ * Units: positions in Angstrom, energies in kcal/mol, temperatures in K.
+ */
+#ifndef REMC_H
+#define REMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define REMC_NTYPES 20
+#define REMC_MAX_BODY_BEADS 1024 /* largest rigid body the MC kernel stages in LDS */
+
+/* status codes */
+#define REMC_OK 0
+#define REMC_ERR_ARG (-1)
+#define REMC_ERR_STATE (-2)
+#define REMC_ERR_HIP (-3)
+#define REMC_ERR_ALLOC (-4)
+#define REMC_ERR_LIMIT (-5)
+#define REMC_ERR_DEVICE (-6)
+
+#if defined(__GNUC__)
+#define REMC_API __attribute__((visibility("default")))
+#else
+#define REMC_API
+#endif
+
+typedef struct remc_ctx remc_ctx;
+
+/* Force field.  E_ij = lambda*(table_ij - e0) in kT units; contact term
+ * LJ = -4 E_ij x6 (x6 - 1), x6 = ((sigma_i+sigma_j)/2r)^6, flipped to
+ * -LJ + 2E_ij when E_ij > 0 and r < 2^(1/6) sigma_ij; Debye-Hueckel
+ * DH = q_i q_j exp(-r/xi)/r; crowder pairs (6/r)^6 below 15 A;
+ * E = (sum LJ * lj_cf + sum DH * dh_cf) * kbt_cf.
+ * Replaces copyLJPotentialDataToDevice/bindLJTexture (cudaExterns.h:14,44)
+ * and the compile-time constants of definitions.h:11-35,61,89. */
+typedef struct remc_forcefield {
+    float table[REMC_NTYPES * REMC_NTYPES]; /* symmetric, row-major */
+    double lambda, e0, xi;
+    double lj_cf, dh_cf, kbt_cf;
+    double crowder_sigma, crowder_cutoff;
+} remc_forcefield;
+
+/* One system of rigid bodies.  Beads are ordered by body; body b owns beads
+ * [body_start[b], body_start[b+1]).  The conformation given here is the
+ * initial state of every replica added for the system.  Replaces the packing
+ * of ReplicaDataToDevice (Replica.cpp:1308-1417): where the reference builds
+ * AoS float4 pos=(x,y,z,molecule) and meta=(aa,q,vdW,role) padded to a block
+ * multiple, this layout is SoA and unpadded. */
+typedef struct remc_system_desc {
+    int n_beads, n_bodies;
+    const int *body_start;        /* n_bodies + 1 */
+    const int *type;              /* n_beads, 0..19 (ignored for crowders) */
+    const float *charge;          /* n_beads */
+    const float *sigma;           /* n_beads, vdW radius */
+    const unsigned char *crowder; /* n_bodies, 1 = repulsive crowder */
+    float box;                    /* periodic edge (CUDA_setBoxDimension); <= 0: none */
+    const float *x, *y, *z;       /* bead positions */
+    const float *rx, *ry, *rz;    /* positions relative to the body centre */
+    const float *cx, *cy, *cz;    /* body centres (n_bodies) */
+    const double *quat;           /* 4*n_bodies, (w,x,y,z) per body */
+} remc_system_desc;
+
+typedef struct remc_counters {
+    uint64_t accept;       /* dE < 0            (Replica::accept)  */
+    uint64_t accept_boltz; /* Boltzmann accepts (Replica::acceptA) */
+    uint64_t reject;       /* rejections        (Replica::reject)  */
+    uint64_t steps;        /* MC steps taken = RNG step counter    */
+} remc_counters;
+
+typedef struct remc_step_record {
+    int32_t body, move, accepted, pad; /* move 1 = rotate; accepted 0/1/2 */
+    double delta;                      /* dE, kcal/mol */
+} remc_step_record;
+
+/* ------------------------------------------------------------- context --- */
+/* Replaces cudaSetDevice + per-thread init in MCthreadableFunction (main.cpp:899-940). */
+REMC_API int remc_ctx_create(int device, remc_ctx **out);
+/* Replaces Replica::FreeDevice (Replica.cpp:1618-1655) and unbindLJTexture. */
+REMC_API int remc_ctx_destroy(remc_ctx *ctx);
+REMC_API const char *remc_last_error(void);
+REMC_API int remc_set_forcefield(remc_ctx *ctx, const remc_forcefield *ff);
+/* Seed of the counter-based RNG (Philox4x32-10).  Replaces Replica::initRNGs
+ * (Replica.cpp:265-290), whose GSL streams are time-seeded. */
+REMC_API int remc_set_seed(remc_ctx *ctx, uint64_t seed);
+
+/* Adds a system; *sys_id receives its index.  Must precede the first compute call. */
+REMC_API int remc_system_add(remc_ctx *ctx, const remc_system_desc *desc, int *sys_id);
+/* Adds n_rep replicas of a system (Replica::copy, REMCSimulation main.cpp:1161-1188).
+ * temperature[k] is both replica k's start temperature and the k-th rung of the
+ * system's exchange ladder (ascending).  Step sizes stay with the replica, as in
+ * the reference (exchangeReplicas swaps temperatures only).  *first_replica
+ * receives the global index of the first new replica. */
+REMC_API int remc_replicas_add(remc_ctx *ctx, int sys_id, int n_rep, const double *temperature,
+                      const float *translate_step, const double *rotate_step, int *first_replica);
+/* Number of replicas/beads/bodies known to the context. */
+REMC_API int remc_get_sizes(remc_ctx *ctx, int *n_systems, int *n_replicas, int *max_beads);
+
+/* -------------------------------------------------------------- energy --- */
+/* Plain per-replica energy: the reference-shaped tiled kernel (block x block
+ * tiles, lower triangle culled, FP32 partials, host FP64 sum) on the current
+ * device conformation of one replica.  Replaces CUDA_EonDevice
+ * (cudaExterns.h:23; CudaFunctions.cu:304-385) / Replica::EonDevice
+ * (Replica.cpp:1601-1616).  block = 0 picks the reference's rule (32 if
+ * N < 1024 else 64, main.cpp:1746-1747). */
+REMC_API int remc_energy_plain(remc_ctx *ctx, int replica, int block, double *out);
+/* Batched full energy of every replica in one launch: FP32 pair terms with
+ * Kahan sums per thread, FP64 across threads and tiles.  out[n_replicas].
+ * Same quantity as Replica::E() (Replica.cpp:603-745). */
+REMC_API int remc_energy_all(remc_ctx *ctx, double *out);
+/* Sets every replica's tracked energy to remc_energy_all's value. */
+REMC_API int remc_refresh_energies(remc_ctx *ctx);
+/* Energy of pairs between non-crowder bodies only, all replicas; the
+ * extra term of Replica::sample (Replica.cpp:1723-1743). */
+REMC_API int remc_energy_noncrowder_all(remc_ctx *ctx, double *out);
+
+/* ------------------------------------------------------------------ MC --- */
+/* nsteps MC steps on every replica in one persistent launch (one workgroup
+ * per replica): device-side move proposal, dE of the moved body, Metropolis
+ * in FP64.  Replaces the MCSearch loop (Replica.cpp:368-478) with its per-step
+ * MoleculeDataToDevice + EonDevice round trips. */
+REMC_API int remc_mc_steps(remc_ctx *ctx, int nsteps);
+/* The reference protocol for one replica: host proposal, host->device copy
+ * of the moved body, remc_energy_plain, device->host partials, host
+ * Metropolis, copy-back on reject (Replica.cpp:386-474).  Uses the same RNG
+ * stream and move arithmetic as remc_mc_steps. */
+REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int block);
+/* One replica-exchange round over every system's temperature ladder, pairs
+ * (k,k+1) for k = offset, offset+2, ...; the offset alternates per call
+ * starting at 0 (main.cpp:1427-1453).  *n_swaps (may be NULL) receives the
+ * number of swaps. */
+REMC_API int remc_exchange(remc_ctx *ctx, int *n_swaps);
+/* rounds x (steps_per_round MC steps + one exchange round), captured once in a
+ * hipGraph and replayed.  Replaces the REMCSimulation/MCthreadableFunction
+ * lock-step loop (main.cpp:971-1012, 1378-1469). */
+REMC_API int remc_run(remc_ctx *ctx, int rounds, int steps_per_round);
+/* Records per-step decisions for the first max_steps steps after this call
+ * (0 disables). */
+REMC_API int remc_set_recording(remc_ctx *ctx, int max_steps);
+REMC_API int remc_get_records(remc_ctx *ctx, int replica, remc_step_record *out, int n);
+
+/* --------------------------------------------------------------- state --- */
+REMC_API int remc_get_energies(remc_ctx *ctx, double *out);
+REMC_API int remc_get_temperatures(remc_ctx *ctx, double *out);
+REMC_API int remc_get_counters(remc_ctx *ctx, remc_counters *out);
+/* slot_replica[k] for system sys: replica holding the k-th ladder rung. */
+REMC_API int remc_get_ladder(remc_ctx *ctx, int sys_id, int *slot_replica);
+/* Conformation of one replica: bead arrays of the system's size, body arrays of
+ * its body count, quat 4 per body.  Any pointer may be NULL on get. */
+REMC_API int remc_get_state(remc_ctx *ctx, int replica, float *x, float *y, float *z,
+                   float *rx, float *ry, float *rz, float *cx, float *cy, float *cz, double *quat);
+REMC_API int remc_set_state(remc_ctx *ctx, int replica, const float *x, const float *y, const float *z,
+                   const float *rx, const float *ry, const float *rz,
+                   const float *cx, const float *cy, const float *cz, const double *quat);
+
+/* ------------------------------------------------------------- adjoint --- */
+/* Per-body force and torque about the body centre (6 doubles per body:
+ * Fx Fy Fz Tx Ty Tz, kcal/mol/A and kcal/mol/rad, F = -dE/dx) and the
+ * gradient of E with respect to (lambda, e0, xi) for one replica.  No
+ * counterpart in the reference (its gradient-free MC has none). */
+REMC_API int remc_adjoint(remc_ctx *ctx, int replica, double *force_torque, double *dparams);
+
+/* -------------------------------------------------------------- timing --- */
+/* Kernel timing with HIP events recorded on the context's stream around every
+ * launch of a kernel class: 0 = MC, 1 = full energy, 2 = plain energy,
+ * 3 = exchange, 4 = adjoint.  enable resets the accumulators. */
+REMC_API int remc_timing_enable(remc_ctx *ctx, int enable);
+REMC_API int remc_timing_get(remc_ctx *ctx, int kernel_class, double *total_ms, int64_t *launches);
+/* Blocks until the context's stream is idle. */
+REMC_API int remc_synchronize(remc_ctx *ctx);
+/* Selects the MC kernel's workgroup size (0 = autotune on first use; else
+ * 256, 512 or 1024). */
+REMC_API int remc_set_mc_block(remc_ctx *ctx, int block);
+REMC_API int remc_get_mc_block(remc_ctx *ctx, int *block);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,264 @@
+"""ctypes wrapper of liboracle.so, the FP64 CPU restatement of the reference.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+import ctypes
+import math
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+NT = 20
+
+_fp = ctypes.POINTER(ctypes.c_float)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class orc_forcefield(ctypes.Structure):
+    _fields_ = [("table", ctypes.c_double * (NT * NT)), ("lambda_", ctypes.c_double), ("e0", ctypes.c_double),
+                ("xi", ctypes.c_double), ("lj_cf", ctypes.c_double), ("dh_cf", ctypes.c_double),
+                ("kbt_cf", ctypes.c_double), ("crowder_sigma", ctypes.c_double),
+                ("crowder_cutoff", ctypes.c_double), ("eps_r", ctypes.c_double)]
+
+
+class orc_system(ctypes.Structure):
+    _fields_ = [("n_beads", ctypes.c_int), ("n_bodies", ctypes.c_int), ("body_start", _ip), ("type", _ip),
+                ("charge", _dp), ("sigma", _dp), ("crowder", ctypes.POINTER(ctypes.c_ubyte)), ("box", ctypes.c_double)]
+
+
+class orc_replica(ctypes.Structure):
+    _fields_ = [("x", _fp), ("y", _fp), ("z", _fp), ("rx", _fp), ("ry", _fp), ("rz", _fp),
+                ("cx", _fp), ("cy", _fp), ("cz", _fp), ("qw", _dp), ("qx", _dp), ("qy", _dp), ("qz", _dp),
+                ("energy", ctypes.c_double), ("temperature", ctypes.c_double), ("translate_step", ctypes.c_float),
+                ("rot_cos_half", ctypes.c_double), ("rot_sin_half", ctypes.c_double), ("step", ctypes.c_uint64),
+                ("rng_id", ctypes.c_uint32), ("n_accept", ctypes.c_uint64), ("n_accept_boltz", ctypes.c_uint64),
+                ("n_reject", ctypes.c_uint64)]
+
+
+class orc_step_record(ctypes.Structure):
+    _fields_ = [("body", ctypes.c_int32), ("move", ctypes.c_int32), ("accepted", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("delta", ctypes.c_double), ("margin", ctypes.c_double)]
+
+
+RECORD_DTYPE = np.dtype([("body", np.int32), ("move", np.int32), ("accepted", np.int32), ("pad", np.int32),
+                         ("delta", np.float64), ("margin", np.float64)])
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_u01.restype = ctypes.c_double
+        L.orc_u01.argtypes = [ctypes.c_uint32]
+        L.orc_exp.restype = ctypes.c_double
+        L.orc_exp.argtypes = [ctypes.c_double]
+        L.orc_fmodf.restype = ctypes.c_float
+        L.orc_fmodf.argtypes = [ctypes.c_float, ctypes.c_float]
+        L.orc_r_gas.restype = ctypes.c_double
+        L.orc_exchange_uniform.restype = ctypes.c_double
+        L.orc_exchange_uniform.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, _u32p]
+        L.orc_pair_energy.restype = ctypes.c_double
+        L.orc_pair_energy.argtypes = [ctypes.POINTER(orc_forcefield), ctypes.c_double] + \
+            [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int] + \
+            [ctypes.c_double] * 3 + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int] + [_dp, _dp]
+        for name in ("orc_energy_full",):
+            getattr(L, name).restype = ctypes.c_double
+        L.orc_energy_full.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp, _dp, _dp]
+        L.orc_energy_noncrowder.restype = ctypes.c_double
+        L.orc_energy_noncrowder.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp]
+        L.orc_energy_bodies.restype = ctypes.c_double
+        L.orc_energy_bodies.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp,
+                                        ctypes.c_int, ctypes.c_int]
+        L.orc_delta_energy.restype = ctypes.c_double
+        L.orc_delta_energy.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp,
+                                       ctypes.c_int, _fp, _fp, _fp]
+        L.orc_pair_table.restype = None
+        L.orc_pair_table.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp,
+                                     ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_metropolis.restype = ctypes.c_int
+        L.orc_metropolis.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_mc_steps.restype = None
+        L.orc_mc_steps.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield),
+                                   ctypes.POINTER(orc_replica), _u32p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.POINTER(orc_step_record)]
+        L.orc_exchange_round.restype = ctypes.c_int
+        L.orc_exchange_round.argtypes = [ctypes.c_int, _dp, _ip, _dp, ctypes.c_int, ctypes.c_uint64,
+                                         ctypes.c_uint32, _u32p, _ip]
+        L.orc_mc_threaded.restype = ctypes.c_double
+        L.orc_mc_threaded.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield),
+                                      ctypes.POINTER(orc_replica), ctypes.c_int, _u32p, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_int]
+        L.orc_propose.restype = None
+        L.orc_propose.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_replica), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double, ctypes.c_double] + [_fp] * 6 + [_fp, _dp]
+        L.orc_step_words.restype = None
+        L.orc_step_words.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u32p, _u32p]
+        L.orc_philox4x32_10.restype = None
+        L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        _lib = L
+    return _lib
+
+
+def make_ff(ff):
+    """orc_forcefield from a remcdockingsimulator_amd.forcefield.ForceField."""
+    o = orc_forcefield()
+    flat = np.asarray(ff.table, dtype=np.float32).astype(np.float64).reshape(-1)
+    for i in range(NT * NT):
+        o.table[i] = flat[i]
+    o.lambda_, o.e0, o.xi = ff.lambda_, ff.e0, ff.xi
+    o.lj_cf, o.dh_cf, o.kbt_cf = ff.lj_cf, ff.dh_cf, ff.kbt_cf
+    o.crowder_sigma, o.crowder_cutoff, o.eps_r = ff.crowder_sigma, ff.crowder_cutoff, ff.eps_r
+    return o
+
+
+def key_words(seed):
+    return (ctypes.c_uint32 * 2)(seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+
+
+class OracleSystem:
+    """A system (model.SystemArrays) bound to the oracle."""
+
+    def __init__(self, sa, ff):
+        self.L = lib()
+        self.sa = sa
+        self.body_start = np.ascontiguousarray(sa.body_start, dtype=np.int32)
+        self.type = np.ascontiguousarray(sa.type, dtype=np.int32)
+        self.charge = np.ascontiguousarray(sa.charge, dtype=np.float32).astype(np.float64)
+        self.sigma = np.ascontiguousarray(sa.sigma, dtype=np.float32).astype(np.float64)
+        self.crowder = np.ascontiguousarray(sa.crowder, dtype=np.uint8)
+        s = orc_system()
+        s.n_beads, s.n_bodies = sa.n_beads, sa.n_bodies
+        s.body_start = self.body_start.ctypes.data_as(_ip)
+        s.type = self.type.ctypes.data_as(_ip)
+        s.charge = self.charge.ctypes.data_as(_dp)
+        s.sigma = self.sigma.ctypes.data_as(_dp)
+        s.crowder = self.crowder.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))
+        s.box = float(np.float32(sa.box))  # the library stores the box edge as float
+        self.s = s
+        self.ff = make_ff(ff)
+
+    @staticmethod
+    def _xyz(pos):
+        pos = np.asarray(pos, dtype=np.float32)
+        return [np.ascontiguousarray(pos[:, k]) for k in range(3)]
+
+    def energy(self, pos, parts=False):
+        x, y, z = self._xyz(pos)
+        lj, dh = ctypes.c_double(), ctypes.c_double()
+        e = self.L.orc_energy_full(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
+                                   y.ctypes.data_as(_fp), z.ctypes.data_as(_fp), ctypes.byref(lj), ctypes.byref(dh))
+        if parts:
+            return e, lj.value * self.ff.lj_cf * self.ff.kbt_cf, dh.value * self.ff.dh_cf * self.ff.kbt_cf
+        return e
+
+    def energy_noncrowder(self, pos):
+        x, y, z = self._xyz(pos)
+        return self.L.orc_energy_noncrowder(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
+                                            y.ctypes.data_as(_fp), z.ctypes.data_as(_fp))
+
+    def energy_bodies(self, pos, a, b):
+        x, y, z = self._xyz(pos)
+        return self.L.orc_energy_bodies(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
+                                        y.ctypes.data_as(_fp), z.ctypes.data_as(_fp), a, b)
+
+    def delta(self, pos, m, new_body_pos):
+        x, y, z = self._xyz(pos)
+        nx, ny, nz = self._xyz(new_body_pos)
+        return self.L.orc_delta_energy(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
+                                       y.ctypes.data_as(_fp), z.ctypes.data_as(_fp), m, nx.ctypes.data_as(_fp),
+                                       ny.ctypes.data_as(_fp), nz.ctypes.data_as(_fp))
+
+    def pair_table(self, pos, a, b):
+        x, y, z = self._xyz(pos)
+        na = self.body_start[a + 1] - self.body_start[a]
+        nb = self.body_start[b + 1] - self.body_start[b]
+        out = np.zeros((na * nb, 3), dtype=np.float64)
+        self.L.orc_pair_table(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
+                              y.ctypes.data_as(_fp), z.ctypes.data_as(_fp), a, b, out.ctypes.data_as(_dp))
+        return out.reshape(na, nb, 3)
+
+
+class OracleReplica:
+    """Replica state owned by numpy arrays, bound to an orc_replica."""
+
+    def __init__(self, osys, temperature, translate_step, rotate_step, rng_id, pos=None, rel=None, center=None,
+                 quat=None, energy=None, step=0):
+        sa = osys.sa
+        self.osys = osys
+        self.pos = np.array(sa.pos if pos is None else pos, dtype=np.float32)
+        self.rel = np.array(sa.rel if rel is None else rel, dtype=np.float32)
+        self.center = np.array(sa.center if center is None else center, dtype=np.float32)
+        self.quat = np.array(sa.quat if quat is None else quat, dtype=np.float64)
+        self.cols = [np.ascontiguousarray(self.pos[:, k]) for k in range(3)] + \
+                    [np.ascontiguousarray(self.rel[:, k]) for k in range(3)] + \
+                    [np.ascontiguousarray(self.center[:, k]) for k in range(3)] + \
+                    [np.ascontiguousarray(self.quat[:, k]) for k in range(4)]
+        r = orc_replica()
+        names = ["x", "y", "z", "rx", "ry", "rz", "cx", "cy", "cz"]
+        for n, a in zip(names, self.cols[:9]):
+            setattr(r, n, a.ctypes.data_as(_fp))
+        for n, a in zip(["qw", "qx", "qy", "qz"], self.cols[9:]):
+            setattr(r, n, a.ctypes.data_as(_dp))
+        r.temperature = temperature
+        r.translate_step = translate_step
+        # libm cos/sin, as the library's host code computes them
+        r.rot_cos_half = math.cos(rotate_step / 2.0)
+        r.rot_sin_half = math.sin(rotate_step / 2.0)
+        r.step = step
+        r.rng_id = rng_id
+        r.energy = osys.energy(self.pos) if energy is None else energy
+        self.r = r
+
+    def sync_out(self):
+        """Copy the C-side column arrays back to the (n,3) views."""
+        self.pos = np.stack(self.cols[:3], 1)
+        self.rel = np.stack(self.cols[3:6], 1)
+        self.center = np.stack(self.cols[6:9], 1)
+        self.quat = np.stack(self.cols[9:], 1)
+
+    def mc(self, seed, nsteps, mode=0, record=True):
+        rec = np.zeros(nsteps, dtype=RECORD_DTYPE) if record else None
+        lib().orc_mc_steps(ctypes.byref(self.osys.s), ctypes.byref(self.osys.ff), ctypes.byref(self.r),
+                           key_words(seed), nsteps, mode,
+                           rec.ctypes.data_as(ctypes.POINTER(orc_step_record)) if record else None)
+        self.sync_out()
+        return rec
+
+    @property
+    def energy(self):
+        return self.r.energy
+
+
+def mc_threaded(osys, reps, seed, nsteps, mode, threads):
+    arr = (orc_replica * len(reps))(*[r.r for r in reps])
+    t = lib().orc_mc_threaded(ctypes.byref(osys.s), ctypes.byref(osys.ff), arr, len(reps), key_words(seed), nsteps,
+                              mode, threads)
+    for k, r in enumerate(reps):
+        r.r = orc_replica.from_buffer_copy(arr[k])
+        r.sync_out()
+    return t
+
+
+def exchange_round(ladder, slot_replica, energies, offset, round_, system_id, seed):
+    lad = np.ascontiguousarray(ladder, dtype=np.float64)
+    slots = np.ascontiguousarray(slot_replica, dtype=np.int32).copy()
+    en = np.ascontiguousarray(energies, dtype=np.float64)
+    sw = np.zeros(len(slots), dtype=np.int32)
+    n = lib().orc_exchange_round(len(slots), lad.ctypes.data_as(_dp), slots.ctypes.data_as(_ip),
+                                 en.ctypes.data_as(_dp), offset, round_, system_id, key_words(seed),
+                                 sw.ctypes.data_as(_ip))
+    return slots, n, sw

@@ -1,0 +1,182 @@
+/*
+ * remc_oracle.h -- CPU FP64 restatement of the reference's REMC hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in remcdockingsimulator_amd/csrc.  Only tests/, the smoke()
+ * entry point and bench.py's cpu_baseline leg may load it; the product never
+ * links or calls it.
+ *
+ * What it restates (reference paths relative to "simulation engine/src/"):
+ *   orc_pair_terms / orc_energy_full  Replica::E()                 Replica.cpp:603-745
+ *                                     distance()                   Replica.cpp:589-600
+ *                                     crowderPairPotential()       Replica.cpp:584-587
+ *   orc_energy_bodies                 Replica::E(Molecule*,Molecule*) Replica.cpp:793-828
+ *   orc_delta_energy                  E_new - E_old of one moved body (same terms as E())
+ *   orc_propose
+ *                                     Molecule::rotateQ            Molecule.cpp:288-333
+ *                                     Quaternion::normalize/multiply/rotateVector
+ *                                                                  Quaternion.cpp:19-101
+ *                                     Replica::translate           Replica.cpp:333-364
+ *                                     Molecule::setPosition        Molecule.cpp:247-260
+ *   orc_mc_steps                      Replica::MCSearch            Replica.cpp:368-478
+ *   orc_exchange_round                REMCSimulation exchange loop main.cpp:1427-1453
+ *                                     Replica::exchangeReplicas    Replica.cpp:145-159
+ *   orc_philox4x32_10                 Philox4x32-10 (Salmon et al., SC'11); replaces the
+ *                                     reference's time-seeded GSL mt19937 streams
+ *                                     (Replica.cpp:265-290), which are not reproducible.
+ *   orc_exp                           exp(x) by Cody-Waite reduction + Taylor polynomial;
+ *                                     the same algorithm is used on the GPU so that
+ *                                     Metropolis/exchange decisions are bit-identical.
+ *
+ * Pinning (see tests/test_oracle_*.py): the reference's own fixtures
+ * (data/structures/energy.txt, data/pairinteractions/pairs_N.dat with the ten
+ * UIM/ubiquitin conformations of "post processor/data/structures.tgz"),
+ * Random123's published Philox known-answer vectors, closed-form two-bead
+ * energies, rigid-motion/permutation/periodicity invariants, brute-force
+ * long-double sums, libm exp/fmodf, and the Rodrigues rotation formula.
+ */
+#ifndef REMC_ORACLE_H
+#define REMC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_NTYPES 20
+
+/* Force field.  E_ij = lambda * (table_ij - e0)  (definitions.h:33-35, Replica.cpp:710). */
+typedef struct orc_forcefield {
+    double table[ORC_NTYPES * ORC_NTYPES]; /* symmetric 20x20, row-major            */
+    double lambda, e0;                     /* MJ1996: 0.159, -2.27                  */
+    double xi;                             /* Debye length, 10 A (definitions.h:22) */
+    double lj_cf;                          /* LJ_CONVERSION_FACTOR (definitions.h:23) */
+    double dh_cf;                          /* DH_CONVERSION_FACTOR*1.602176487^2     */
+    double kbt_cf;                         /* KBTConversionFactor (definitions.h:18) */
+    double crowder_sigma;                  /* 6 A   (Replica.cpp:586)               */
+    double crowder_cutoff;                 /* 15 A  (definitions.h:89)              */
+    double eps_r;                          /* EPS=1e-38 added to r (definitions.h:61) */
+} orc_forcefield;
+
+/* One system: beads grouped into rigid bodies, bodies [b] = beads
+ * [body_start[b], body_start[b+1]).  Crowder bodies use the repulsive term. */
+typedef struct orc_system {
+    int n_beads, n_bodies;
+    const int *body_start;          /* n_bodies + 1 */
+    const int *type;                /* n_beads, 0..19 */
+    const double *charge;           /* n_beads */
+    const double *sigma;            /* n_beads, vdW radius */
+    const unsigned char *crowder;   /* n_bodies, 1 = crowder (role -1) */
+    double box;                     /* periodic edge L; <= 0 disables minimum image */
+} orc_system;
+
+/* Dynamic state of one replica (float storage as in Residue/Molecule). */
+typedef struct orc_replica {
+    float *x, *y, *z;               /* bead positions        n_beads  */
+    float *rx, *ry, *rz;            /* relative positions    n_beads  */
+    float *cx, *cy, *cz;            /* body centres          n_bodies */
+    double *qw, *qx, *qy, *qz;      /* body rotations        n_bodies */
+    double energy;                  /* current potential, kcal/mol   */
+    double temperature;             /* K                              */
+    float translate_step;           /* A   (fixed magnitude)          */
+    double rot_cos_half, rot_sin_half; /* cos/sin of rotateStep/2      */
+    uint64_t step;                  /* MC step counter = RNG counter  */
+    uint32_t rng_id;                /* replica id inside the RNG counter */
+    uint64_t n_accept, n_accept_boltz, n_reject;
+} orc_replica;
+
+/* Per-step record for decision parity. */
+typedef struct orc_step_record {
+    int32_t body;
+    int32_t move;        /* 1 = rotate, 0 = translate */
+    int32_t accepted;    /* 0 reject, 1 accept (dE<0), 2 accept (Boltzmann) */
+    int32_t pad;
+    double delta;        /* dE, kcal/mol */
+    double margin;       /* distance of the decision from its threshold: |dE| if dE<0,
+                            else log(u) + beta*dE (accept if < 0)             */
+} orc_step_record;
+
+/* constants ---------------------------------------------------------------- */
+void orc_default_constants(orc_forcefield *ff); /* float-derived reference constants */
+double orc_r_gas(void);                          /* Rgas = 8.314472f (definitions.h:16) */
+
+/* RNG ---------------------------------------------------------------------- */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double orc_u01(uint32_t w);
+/* The six draws of one MC step: [0] body word, [1] move word, [2..4] vector
+ * components words, [5] Metropolis word. */
+void orc_step_words(uint64_t step, uint32_t rng_id, const uint32_t key[2], uint32_t w[6]);
+double orc_exchange_uniform(uint64_t round, uint32_t pair, uint32_t system_id, const uint32_t key[2]);
+
+/* math --------------------------------------------------------------------- */
+double orc_exp(double x);
+float orc_fmodf(float x, float L);
+
+/* energy ------------------------------------------------------------------- */
+/* Pair energy in kcal/mol for two beads with explicit parameters; lj_raw/dh_raw
+ * receive the unconverted LJ and DH values (may be NULL). */
+double orc_pair_energy(const orc_forcefield *ff, double L,
+                       double xi_, double yi_, double zi_, int ti, double qi, double si, int ci,
+                       double xj_, double yj_, double zj_, int tj, double qj, double sj, int cj,
+                       double *lj_raw, double *dh_raw);
+/* Full energy: returns kcal/mol; lj_sum/dh_sum get the raw accumulators (may be NULL). */
+double orc_energy_full(const orc_system *s, const orc_forcefield *ff,
+                       const float *x, const float *y, const float *z,
+                       double *lj_sum, double *dh_sum);
+/* Energy restricted to pairs between non-crowder bodies (sampling, Replica.cpp:1723-1743). */
+double orc_energy_noncrowder(const orc_system *s, const orc_forcefield *ff,
+                             const float *x, const float *y, const float *z);
+/* Energy between two bodies a != b, kcal/mol. */
+double orc_energy_bodies(const orc_system *s, const orc_forcefield *ff,
+                         const float *x, const float *y, const float *z, int a, int b);
+/* dE of moving body m from (x,y,z) to the body positions (nx,ny,nz)[0..n_m). */
+double orc_delta_energy(const orc_system *s, const orc_forcefield *ff,
+                        const float *x, const float *y, const float *z, int m,
+                        const float *nx, const float *ny, const float *nz);
+/* Per-pair energies between bodies a and b in order (i in a, j in b): out[k] = (tot, dh, lj)
+ * in kcal/mol, k = (i-start_a)*n_b + (j-start_b). */
+void orc_pair_table(const orc_system *s, const orc_forcefield *ff,
+                    const float *x, const float *y, const float *z, int a, int b, double *out);
+
+/* moves -------------------------------------------------------------------- */
+void orc_quat_mul(const double a[4], const double b[4], double out[4]);
+void orc_quat_rotate(const double q[4], const float v[3], float out[3]);
+/* Proposal of one rigid move of body m (no state change).
+ *   rotate=1: axis = normalize(a,b,c) in FP64, q = (ch, sh*axis) normalised,
+ *             rel' = q.rotateVector(rel) (FP64 math, float storage),
+ *             rotation' = q * rotation, centre unchanged.
+ *   rotate=0: t = step * normalize((float)a,(float)b,(float)c) in FP32,
+ *             centre' = fmod(centre + t + L, L) per axis, rel unchanged.
+ *   In both: pos' = rel' + centre' (FP32).
+ * Outputs have body_size entries (nx..nrz) or one body (cnew[3], qnew[4]). */
+void orc_propose(const orc_system *s, const orc_replica *r, int m, int rotate,
+                 double a, double b, double c,
+                 float *nx, float *ny, float *nz, float *nrx, float *nry, float *nrz,
+                 float cnew[3], double qnew[4]);
+
+/* MC ------------------------------------------------------------------------ */
+/* mode 0: dE of the moved body (FP64); mode 1: full recomputation each step
+ * (the reference CPU path, Replica.cpp:435).  rec may be NULL. */
+void orc_mc_steps(const orc_system *s, const orc_forcefield *ff, orc_replica *r,
+                  const uint32_t key[2], int nsteps, int mode, orc_step_record *rec);
+/* Metropolis test shared by both modes (Replica.cpp:438-474, FP64). */
+int orc_metropolis(double delta, double temperature, double u, double *margin);
+
+/* Exchange over temperature slots.  slot_replica[k] = replica holding the k-th
+ * temperature of the sorted ladder ladder[k]; energies by replica index.
+ * Swaps accepted pairs in slot_replica; returns the number of swaps; swapped[k]
+ * (may be NULL) = 1 if pair (k,k+1) swapped. */
+int orc_exchange_round(int n_rep, const double *ladder, int *slot_replica,
+                       const double *energy, int offset, uint64_t round, uint32_t system_id,
+                       const uint32_t key[2], int *swapped);
+
+/* Threaded CPU baseline: runs nsteps on each replica with block-partitioned
+ * pthreads (main.cpp:879-889).  Returns wall seconds. */
+double orc_mc_threaded(const orc_system *s, const orc_forcefield *ff, orc_replica *reps,
+                       int n_rep, const uint32_t key[2], int nsteps, int mode, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1084 @@
+// remc_api.hip -- implementation of the C ABI declared in include/remc.h.
+//
+// Host-side responsibilities: validate inputs, lay systems and replicas out in
+// HBM (see remc_internal.h), launch kernels on the context's stream, run the
+// reference's per-step protocol for the plain path, capture MC+exchange rounds
+// in a hipGraph, and time launches with HIP events.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "remc.h"
+#include "remc_internal.h"
+#include "remc_math.h"
+
+using namespace remc;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) return fail(REMC_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+struct HostSys {
+    int n_beads, n_bodies;
+    std::vector<int> body_start, type, crowder;
+    std::vector<float> charge, sigma;
+    float box;
+    std::vector<float> x, y, z, rx, ry, rz, cx, cy, cz;
+    std::vector<double> quat;
+    std::vector<int> replicas;    // global replica ids, ladder order
+    std::vector<double> ladder;   // ascending temperatures
+    int max_body;
+};
+
+struct HostRep {
+    int sys;
+    double T;
+    float tstep;
+    double rstep;
+};
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n)
+{
+    *p = nullptr;
+    if (n == 0) n = 1;
+    return hipMalloc((void **)p, n * sizeof(T));
+}
+
+template <typename T>
+hipError_t upload(T *d, const std::vector<T> &h, hipStream_t s)
+{
+    if (h.empty()) return hipSuccess;
+    return hipMemcpyAsync(d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+}
+
+} // namespace
+
+struct remc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    remc_forcefield ff{};
+    bool ff_set = false;
+    uint64_t seed = 0x52454D43ull;
+    std::vector<HostSys> systems;
+    std::vector<HostRep> reps;
+    bool finalized = false;
+
+    // sizes
+    int total_static_beads = 0, total_static_bodies = 0;
+    int total_rep_beads = 0, total_rep_bodies = 0, total_slots = 0;
+    int max_beads = 0, max_tiles = 0;
+    std::vector<int> rep_bead_off, rep_body_off;
+
+    // device buffers
+    int *d_tc = nullptr, *d_body = nullptr, *d_bstart = nullptr, *d_bend = nullptr, *d_bcrow = nullptr;
+    float *d_q = nullptr, *d_sig = nullptr, *d_etab = nullptr;
+    DevSys *d_sys = nullptr;
+    DevRep *d_rep = nullptr;
+    float *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_rx = nullptr, *d_ry = nullptr, *d_rz = nullptr;
+    float *d_cx = nullptr, *d_cy = nullptr, *d_cz = nullptr;
+    double *d_quat = nullptr, *d_E = nullptr, *d_T = nullptr;
+    unsigned long long *d_step = nullptr, *d_cnt = nullptr;
+    int *d_slot = nullptr, *d_exoff = nullptr;
+    double *d_ladder = nullptr;
+    unsigned long long *d_exround = nullptr, *d_exswaps = nullptr;
+    double *d_full_partials = nullptr, *d_out = nullptr;
+    float *d_plain_partials = nullptr;
+    size_t plain_partials_cap = 0;
+    remc_step_record *d_rec = nullptr;
+    unsigned long long *d_rec_start = nullptr;
+    int rec_max = 0;
+    DevFF devff{};
+
+    // MC configuration
+    int mc_block = 0;
+
+    // graph
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = -1;
+
+    // timing
+    bool timing = false;
+    struct EvPair {
+        int cls;
+        hipEvent_t a, b;
+    };
+    std::vector<EvPair> pending;
+    std::vector<hipEvent_t> free_events;
+    double t_total[5] = {0, 0, 0, 0, 0};
+    int64_t t_count[5] = {0, 0, 0, 0, 0};
+
+    KParams params() const
+    {
+        KParams p{};
+        p.tc = d_tc; p.q = d_q; p.sig = d_sig; p.body = d_body;
+        p.bstart = d_bstart; p.bend = d_bend; p.bcrow = d_bcrow;
+        p.sys = d_sys; p.rep = d_rep; p.etab = d_etab; p.ff = devff;
+        p.px = d_px; p.py = d_py; p.pz = d_pz; p.rx = d_rx; p.ry = d_ry; p.rz = d_rz;
+        p.cx = d_cx; p.cy = d_cy; p.cz = d_cz; p.quat = d_quat;
+        p.E = d_E; p.T = d_T; p.step = d_step; p.cnt = d_cnt;
+        p.slot_rep = d_slot; p.ladder = d_ladder; p.ex_round = d_exround; p.ex_offset = d_exoff;
+        p.ex_swaps = d_exswaps;
+        p.rec = rec_max > 0 ? d_rec : nullptr;
+        p.rec_start = d_rec_start;
+        p.rec_max = rec_max;
+        p.key0 = (uint32_t)seed;
+        p.key1 = (uint32_t)(seed >> 32);
+        return p;
+    }
+};
+
+namespace {
+
+hipEvent_t get_event(remc_ctx *c)
+{
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct TimedLaunch {
+    remc_ctx *c;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedLaunch(remc_ctx *c_, int cls_) : c(c_), cls(cls_)
+    {
+        if (c->timing) {
+            a = get_event(c);
+            b = get_event(c);
+            if (a && b) hipEventRecord(a, c->stream);
+        }
+    }
+    ~TimedLaunch()
+    {
+        if (a && b) {
+            hipEventRecord(b, c->stream);
+            c->pending.push_back({cls, a, b});
+        }
+    }
+};
+
+int collect_timing(remc_ctx *c)
+{
+    for (auto &pe : c->pending) {
+        HIP_TRY(hipEventSynchronize(pe.b));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, pe.a, pe.b));
+        c->t_total[pe.cls] += ms;
+        c->t_count[pe.cls] += 1;
+        c->free_events.push_back(pe.a);
+        c->free_events.push_back(pe.b);
+    }
+    c->pending.clear();
+    return REMC_OK;
+}
+
+void destroy_graph(remc_ctx *c)
+{
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_steps = -1;
+}
+
+int finalize(remc_ctx *c)
+{
+    if (c->finalized) return REMC_OK;
+    if (!c->ff_set) return fail(REMC_ERR_STATE, "force field not set");
+    if (c->systems.empty()) return fail(REMC_ERR_STATE, "no systems added");
+    if (c->reps.empty()) return fail(REMC_ERR_STATE, "no replicas added");
+    HIP_TRY(hipSetDevice(c->device));
+
+    // static layout
+    std::vector<int> tc, body, bstart, bend, bcrow;
+    std::vector<float> q, sig;
+    std::vector<DevSys> dsys;
+    int bead_off = 0, body_off = 0, slot_off = 0;
+    for (auto &s : c->systems) {
+        DevSys d{};
+        d.bead_off = bead_off;
+        d.n_beads = s.n_beads;
+        d.body_off = body_off;
+        d.n_bodies = s.n_bodies;
+        d.max_body = s.max_body;
+        d.rep_first = s.replicas.empty() ? 0 : s.replicas.front();
+        d.n_rep = (int)s.replicas.size();
+        d.slot_off = slot_off;
+        d.box = s.box > 0.f ? s.box : 0.f;
+        d.inv_box = s.box > 0.f ? 1.0f / s.box : 0.f;
+        dsys.push_back(d);
+        for (int m = 0; m < s.n_bodies; ++m) {
+            bstart.push_back(s.body_start[m]);
+            bend.push_back(s.body_start[m + 1]);
+            bcrow.push_back(s.crowder[m]);
+            for (int b = s.body_start[m]; b < s.body_start[m + 1]; ++b) {
+                tc.push_back((s.type[b] & 0xff) | (s.crowder[m] ? 0x100 : 0));
+                body.push_back(m);
+                q.push_back(s.charge[b]);
+                sig.push_back(s.sigma[b]);
+            }
+        }
+        bead_off += s.n_beads;
+        body_off += s.n_bodies;
+        slot_off += (int)s.replicas.size();
+        c->max_beads = std::max(c->max_beads, s.n_beads);
+    }
+    c->total_static_beads = bead_off;
+    c->total_static_bodies = body_off;
+    c->total_slots = slot_off;
+
+    // replica layout
+    const int R = (int)c->reps.size();
+    std::vector<DevRep> drep(R);
+    c->rep_bead_off.assign(R, 0);
+    c->rep_body_off.assign(R, 0);
+    std::vector<float> px, py, pz, rx, ry, rz, cx, cy, cz;
+    std::vector<double> quat, E(R, 0.0), T(R);
+    int rbo = 0, rmo = 0;
+    for (int r = 0; r < R; ++r) {
+        const HostRep &h = c->reps[r];
+        const HostSys &s = c->systems[h.sys];
+        DevRep d{};
+        d.sys = h.sys;
+        d.bead_off = rbo;
+        d.body_off = rmo;
+        d.rng_id = (uint32_t)r;
+        d.tstep = h.tstep;
+        d.rot_ch = std::cos(h.rstep / 2.0);
+        d.rot_sh = std::sin(h.rstep / 2.0);
+        drep[r] = d;
+        c->rep_bead_off[r] = rbo;
+        c->rep_body_off[r] = rmo;
+        px.insert(px.end(), s.x.begin(), s.x.end());
+        py.insert(py.end(), s.y.begin(), s.y.end());
+        pz.insert(pz.end(), s.z.begin(), s.z.end());
+        rx.insert(rx.end(), s.rx.begin(), s.rx.end());
+        ry.insert(ry.end(), s.ry.begin(), s.ry.end());
+        rz.insert(rz.end(), s.rz.begin(), s.rz.end());
+        cx.insert(cx.end(), s.cx.begin(), s.cx.end());
+        cy.insert(cy.end(), s.cy.begin(), s.cy.end());
+        cz.insert(cz.end(), s.cz.begin(), s.cz.end());
+        quat.insert(quat.end(), s.quat.begin(), s.quat.end());
+        T[r] = h.T;
+        rbo += s.n_beads;
+        rmo += s.n_bodies;
+    }
+    c->total_rep_beads = rbo;
+    c->total_rep_bodies = rmo;
+    std::vector<int> slots;
+    std::vector<double> ladder;
+    for (auto &s : c->systems) {
+        slots.insert(slots.end(), s.replicas.begin(), s.replicas.end());
+        ladder.insert(ladder.end(), s.ladder.begin(), s.ladder.end());
+    }
+
+    // force field
+    std::vector<float> etab(REMC_NTYPES * REMC_NTYPES);
+    const remc_forcefield &f = c->ff;
+    const double lj_scale = f.lj_cf * f.kbt_cf;
+    for (int i = 0; i < REMC_NTYPES * REMC_NTYPES; ++i)
+        etab[i] = (float)(f.lambda * ((double)f.table[i] - f.e0) * lj_scale);
+    c->devff.c_crow = (float)lj_scale;
+    c->devff.crow_s2 = (float)(f.crowder_sigma * f.crowder_sigma);
+    c->devff.crow_cut2 = (float)(f.crowder_cutoff * f.crowder_cutoff);
+    c->devff.c_dh = (float)(f.dh_cf * f.kbt_cf);
+    c->devff.k_dh = (float)(-1.4426950408889634 / f.xi);
+    c->devff.eps_r = 1e-38f;
+
+    c->max_tiles = (c->max_beads + FULL_TILE - 1) / FULL_TILE;
+
+    hipStream_t s = c->stream;
+    HIP_TRY(dalloc(&c->d_tc, tc.size()));
+    HIP_TRY(dalloc(&c->d_body, body.size()));
+    HIP_TRY(dalloc(&c->d_q, q.size()));
+    HIP_TRY(dalloc(&c->d_sig, sig.size()));
+    HIP_TRY(dalloc(&c->d_bstart, bstart.size()));
+    HIP_TRY(dalloc(&c->d_bend, bend.size()));
+    HIP_TRY(dalloc(&c->d_bcrow, bcrow.size()));
+    HIP_TRY(dalloc(&c->d_sys, dsys.size()));
+    HIP_TRY(dalloc(&c->d_rep, drep.size()));
+    HIP_TRY(dalloc(&c->d_etab, etab.size()));
+    HIP_TRY(dalloc(&c->d_px, px.size()));
+    HIP_TRY(dalloc(&c->d_py, py.size()));
+    HIP_TRY(dalloc(&c->d_pz, pz.size()));
+    HIP_TRY(dalloc(&c->d_rx, rx.size()));
+    HIP_TRY(dalloc(&c->d_ry, ry.size()));
+    HIP_TRY(dalloc(&c->d_rz, rz.size()));
+    HIP_TRY(dalloc(&c->d_cx, cx.size()));
+    HIP_TRY(dalloc(&c->d_cy, cy.size()));
+    HIP_TRY(dalloc(&c->d_cz, cz.size()));
+    HIP_TRY(dalloc(&c->d_quat, quat.size()));
+    HIP_TRY(dalloc(&c->d_E, (size_t)R));
+    HIP_TRY(dalloc(&c->d_T, (size_t)R));
+    HIP_TRY(dalloc(&c->d_step, (size_t)R));
+    HIP_TRY(dalloc(&c->d_cnt, (size_t)3 * R));
+    HIP_TRY(dalloc(&c->d_slot, slots.size()));
+    HIP_TRY(dalloc(&c->d_ladder, ladder.size()));
+    HIP_TRY(dalloc(&c->d_exround, c->systems.size()));
+    HIP_TRY(dalloc(&c->d_exoff, c->systems.size()));
+    HIP_TRY(dalloc(&c->d_exswaps, c->systems.size()));
+    HIP_TRY(dalloc(&c->d_full_partials, (size_t)R * c->max_tiles));
+    HIP_TRY(dalloc(&c->d_out, (size_t)R));
+    HIP_TRY(dalloc(&c->d_rec_start, (size_t)R));
+
+    HIP_TRY(upload(c->d_tc, tc, s));
+    HIP_TRY(upload(c->d_body, body, s));
+    HIP_TRY(upload(c->d_q, q, s));
+    HIP_TRY(upload(c->d_sig, sig, s));
+    HIP_TRY(upload(c->d_bstart, bstart, s));
+    HIP_TRY(upload(c->d_bend, bend, s));
+    HIP_TRY(upload(c->d_bcrow, bcrow, s));
+    HIP_TRY(upload(c->d_sys, dsys, s));
+    HIP_TRY(upload(c->d_rep, drep, s));
+    HIP_TRY(upload(c->d_etab, etab, s));
+    HIP_TRY(upload(c->d_px, px, s));
+    HIP_TRY(upload(c->d_py, py, s));
+    HIP_TRY(upload(c->d_pz, pz, s));
+    HIP_TRY(upload(c->d_rx, rx, s));
+    HIP_TRY(upload(c->d_ry, ry, s));
+    HIP_TRY(upload(c->d_rz, rz, s));
+    HIP_TRY(upload(c->d_cx, cx, s));
+    HIP_TRY(upload(c->d_cy, cy, s));
+    HIP_TRY(upload(c->d_cz, cz, s));
+    HIP_TRY(upload(c->d_quat, quat, s));
+    HIP_TRY(upload(c->d_E, E, s));
+    HIP_TRY(upload(c->d_T, T, s));
+    HIP_TRY(upload(c->d_slot, slots, s));
+    HIP_TRY(upload(c->d_ladder, ladder, s));
+    std::vector<unsigned long long> zeros_r((size_t)3 * R, 0ull), zeros_s(c->systems.size(), 0ull);
+    std::vector<int> zeros_si(c->systems.size(), 0);
+    HIP_TRY(hipMemcpyAsync(c->d_step, zeros_r.data(), sizeof(unsigned long long) * R, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_cnt, zeros_r.data(), sizeof(unsigned long long) * 3 * R, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_rec_start, zeros_r.data(), sizeof(unsigned long long) * R, hipMemcpyHostToDevice, s));
+    HIP_TRY(upload(c->d_exround, zeros_s, s));
+    HIP_TRY(upload(c->d_exswaps, zeros_s, s));
+    HIP_TRY(upload(c->d_exoff, zeros_si, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    c->finalized = true;
+
+    // initial energies of every replica
+    KParams p = c->params();
+    HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return REMC_OK;
+}
+
+#define CHECK_CTX(c)                                                   \
+    do {                                                               \
+        if (!(c)) return fail(REMC_ERR_ARG, "null context");           \
+    } while (0)
+
+#define FINALIZE(c)                         \
+    do {                                    \
+        int _rc = finalize(c);              \
+        if (_rc != REMC_OK) return _rc;     \
+    } while (0)
+
+#define CHECK_REP(c, r)                                                                          \
+    do {                                                                                         \
+        if ((r) < 0 || (r) >= (int)(c)->reps.size()) return fail(REMC_ERR_ARG, "replica %d out of range", (r)); \
+    } while (0)
+
+int autotune_mc(remc_ctx *c)
+{
+    // time each workgroup size on the live state, then restore it exactly
+    const int R = (int)c->reps.size();
+    struct Buf {
+        void *dev;
+        size_t bytes;
+        void *save;
+    };
+    std::vector<Buf> bufs = {
+        {c->d_px, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_py, sizeof(float) * c->total_rep_beads, nullptr},
+        {c->d_pz, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_rx, sizeof(float) * c->total_rep_beads, nullptr},
+        {c->d_ry, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_rz, sizeof(float) * c->total_rep_beads, nullptr},
+        {c->d_cx, sizeof(float) * c->total_rep_bodies, nullptr}, {c->d_cy, sizeof(float) * c->total_rep_bodies, nullptr},
+        {c->d_cz, sizeof(float) * c->total_rep_bodies, nullptr}, {c->d_quat, sizeof(double) * 4 * c->total_rep_bodies, nullptr},
+        {c->d_E, sizeof(double) * R, nullptr}, {c->d_step, sizeof(unsigned long long) * R, nullptr},
+        {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}};
+    for (auto &b : bufs) {
+        HIP_TRY(hipMalloc(&b.save, b.bytes));
+        HIP_TRY(hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    KParams p = c->params();
+    p.rec = nullptr;
+    const int cands[3] = {256, 512, 1024};
+    double best = 1e30;
+    int best_block = 256;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    int trial_steps = 64;
+    for (int ci = 0; ci < 3; ++ci) {
+        const int blk = cands[ci];
+        HIP_TRY(launch_mc(p, R, 4, blk, c->stream)); // warm
+        HIP_TRY(hipEventRecord(e0, c->stream));
+        HIP_TRY(launch_mc(p, R, trial_steps, blk, c->stream));
+        HIP_TRY(hipEventRecord(e1, c->stream));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_block = blk;
+        }
+        for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (auto &b : bufs) hipFree(b.save);
+    c->mc_block = best_block;
+    return REMC_OK;
+}
+
+int ensure_mc_block(remc_ctx *c)
+{
+    if (c->mc_block == 0) return autotune_mc(c);
+    return REMC_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *remc_last_error(void) { return g_last_error.c_str(); }
+
+int remc_ctx_create(int device, remc_ctx **out)
+{
+    if (!out) return fail(REMC_ERR_ARG, "null output pointer");
+    *out = nullptr;
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) return fail(REMC_ERR_DEVICE, "no HIP device available");
+    if (device < 0 || device >= n) return fail(REMC_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    HIP_TRY(hipSetDevice(device));
+    remc_ctx *c = new remc_ctx();
+    c->device = device;
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(REMC_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return REMC_OK;
+}
+
+int remc_ctx_destroy(remc_ctx *c)
+{
+    if (!c) return REMC_OK;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    collect_timing(c);
+    destroy_graph(c);
+    void *ptrs[] = {c->d_tc, c->d_body, c->d_bstart, c->d_bend, c->d_bcrow, c->d_q, c->d_sig, c->d_etab,
+                    c->d_sys, c->d_rep, c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz,
+                    c->d_cx, c->d_cy, c->d_cz, c->d_quat, c->d_E, c->d_T, c->d_step, c->d_cnt,
+                    c->d_slot, c->d_exoff, c->d_ladder, c->d_exround, c->d_exswaps, c->d_full_partials,
+                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    for (auto e : c->free_events) hipEventDestroy(e);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return REMC_OK;
+}
+
+int remc_set_forcefield(remc_ctx *c, const remc_forcefield *ff)
+{
+    CHECK_CTX(c);
+    if (!ff) return fail(REMC_ERR_ARG, "null force field");
+    if (c->finalized) return fail(REMC_ERR_STATE, "force field must be set before the first compute call");
+    if (!(ff->xi > 0.0)) return fail(REMC_ERR_ARG, "Debye length must be positive");
+    for (int i = 0; i < REMC_NTYPES; ++i)
+        for (int j = 0; j < REMC_NTYPES; ++j)
+            if (ff->table[i * REMC_NTYPES + j] != ff->table[j * REMC_NTYPES + i])
+                return fail(REMC_ERR_ARG, "pair table is not symmetric at (%d,%d)", i, j);
+    c->ff = *ff;
+    c->ff_set = true;
+    return REMC_OK;
+}
+
+int remc_set_seed(remc_ctx *c, uint64_t seed)
+{
+    CHECK_CTX(c);
+    c->seed = seed;
+    destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_system_add(remc_ctx *c, const remc_system_desc *d, int *sys_id)
+{
+    CHECK_CTX(c);
+    if (!d) return fail(REMC_ERR_ARG, "null system description");
+    if (c->finalized) return fail(REMC_ERR_STATE, "systems must be added before the first compute call");
+    if (d->n_beads <= 0 || d->n_bodies <= 0) return fail(REMC_ERR_ARG, "empty system");
+    if (!d->body_start || !d->type || !d->charge || !d->sigma || !d->crowder || !d->x || !d->y || !d->z || !d->rx ||
+        !d->ry || !d->rz || !d->cx || !d->cy || !d->cz || !d->quat)
+        return fail(REMC_ERR_ARG, "null array in system description");
+    if (d->body_start[0] != 0 || d->body_start[d->n_bodies] != d->n_beads)
+        return fail(REMC_ERR_ARG, "body_start must run from 0 to n_beads");
+    HostSys s;
+    s.n_beads = d->n_beads;
+    s.n_bodies = d->n_bodies;
+    s.max_body = 0;
+    for (int m = 0; m < d->n_bodies; ++m) {
+        int n = d->body_start[m + 1] - d->body_start[m];
+        if (n <= 0) return fail(REMC_ERR_ARG, "body %d is empty or body_start is not increasing", m);
+        if (n > REMC_MAX_BODY_BEADS)
+            return fail(REMC_ERR_LIMIT, "body %d has %d beads, above the limit %d", m, n, REMC_MAX_BODY_BEADS);
+        s.max_body = std::max(s.max_body, n);
+    }
+    for (int b = 0; b < d->n_beads; ++b) {
+        if (d->type[b] < 0 || d->type[b] >= REMC_NTYPES) {
+            // crowder beads carry no type (the reference sets aa index -1)
+            int m = (int)(std::upper_bound(d->body_start, d->body_start + d->n_bodies + 1, b) - d->body_start) - 1;
+            if (!d->crowder[m]) return fail(REMC_ERR_ARG, "bead %d has type %d outside 0..19", b, d->type[b]);
+        }
+        if (!std::isfinite(d->x[b]) || !std::isfinite(d->y[b]) || !std::isfinite(d->z[b]) ||
+            !std::isfinite(d->rx[b]) || !std::isfinite(d->ry[b]) || !std::isfinite(d->rz[b]))
+            return fail(REMC_ERR_ARG, "bead %d has a non-finite position", b);
+    }
+    for (int m = 0; m < d->n_bodies; ++m) {
+        if (!std::isfinite(d->cx[m]) || !std::isfinite(d->cy[m]) || !std::isfinite(d->cz[m]))
+            return fail(REMC_ERR_ARG, "body %d has a non-finite centre", m);
+        for (int k = 0; k < 4; ++k)
+            if (!std::isfinite(d->quat[4 * m + k])) return fail(REMC_ERR_ARG, "body %d has a non-finite rotation", m);
+    }
+    if (!std::isfinite(d->box)) return fail(REMC_ERR_ARG, "box edge is not finite");
+    s.body_start.assign(d->body_start, d->body_start + d->n_bodies + 1);
+    s.type.resize(d->n_beads);
+    for (int b = 0; b < d->n_beads; ++b) s.type[b] = (d->type[b] >= 0 && d->type[b] < REMC_NTYPES) ? d->type[b] : 0;
+    s.crowder.resize(d->n_bodies);
+    for (int m = 0; m < d->n_bodies; ++m) s.crowder[m] = d->crowder[m] ? 1 : 0;
+    s.charge.assign(d->charge, d->charge + d->n_beads);
+    s.sigma.assign(d->sigma, d->sigma + d->n_beads);
+    s.box = d->box;
+    s.x.assign(d->x, d->x + d->n_beads);
+    s.y.assign(d->y, d->y + d->n_beads);
+    s.z.assign(d->z, d->z + d->n_beads);
+    s.rx.assign(d->rx, d->rx + d->n_beads);
+    s.ry.assign(d->ry, d->ry + d->n_beads);
+    s.rz.assign(d->rz, d->rz + d->n_beads);
+    s.cx.assign(d->cx, d->cx + d->n_bodies);
+    s.cy.assign(d->cy, d->cy + d->n_bodies);
+    s.cz.assign(d->cz, d->cz + d->n_bodies);
+    s.quat.assign(d->quat, d->quat + 4 * d->n_bodies);
+    c->systems.push_back(std::move(s));
+    if (sys_id) *sys_id = (int)c->systems.size() - 1;
+    return REMC_OK;
+}
+
+int remc_replicas_add(remc_ctx *c, int sys_id, int n_rep, const double *temperature, const float *translate_step,
+                      const double *rotate_step, int *first_replica)
+{
+    CHECK_CTX(c);
+    if (c->finalized) return fail(REMC_ERR_STATE, "replicas must be added before the first compute call");
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "system %d out of range", sys_id);
+    if (n_rep <= 0 || !temperature || !translate_step || !rotate_step) return fail(REMC_ERR_ARG, "bad replica arguments");
+    HostSys &s = c->systems[sys_id];
+    if (!s.replicas.empty()) return fail(REMC_ERR_STATE, "system %d already has replicas", sys_id);
+    for (int k = 0; k < n_rep; ++k) {
+        if (!(temperature[k] > 0.0)) return fail(REMC_ERR_ARG, "temperature %d must be positive", k);
+        if (k > 0 && !(temperature[k] > temperature[k - 1]))
+            return fail(REMC_ERR_ARG, "temperature ladder must be strictly ascending");
+    }
+    int first = (int)c->reps.size();
+    for (int k = 0; k < n_rep; ++k) {
+        c->reps.push_back({sys_id, temperature[k], translate_step[k], rotate_step[k]});
+        s.replicas.push_back(first + k);
+        s.ladder.push_back(temperature[k]);
+    }
+    if (first_replica) *first_replica = first;
+    return REMC_OK;
+}
+
+int remc_get_sizes(remc_ctx *c, int *n_systems, int *n_replicas, int *max_beads)
+{
+    CHECK_CTX(c);
+    if (n_systems) *n_systems = (int)c->systems.size();
+    if (n_replicas) *n_replicas = (int)c->reps.size();
+    if (max_beads) {
+        int m = 0;
+        for (auto &s : c->systems) m = std::max(m, s.n_beads);
+        *max_beads = m;
+    }
+    return REMC_OK;
+}
+
+int remc_energy_plain(remc_ctx *c, int replica, int block, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    const HostSys &s = c->systems[c->reps[replica].sys];
+    if (block == 0) block = s.n_beads < 1024 ? 32 : 64;
+    if (block != 32 && block != 64 && block != 128 && block != 256)
+        return fail(REMC_ERR_ARG, "plain block size must be 32, 64, 128 or 256");
+    const int G = (s.n_beads + block - 1) / block;
+    const size_t need = (size_t)G * G;
+    if (need > c->plain_partials_cap) {
+        if (c->d_plain_partials) hipFree(c->d_plain_partials);
+        c->d_plain_partials = nullptr;
+        HIP_TRY(hipMalloc(&c->d_plain_partials, need * sizeof(float)));
+        c->plain_partials_cap = need;
+    }
+    // blocks of the culled lower triangle never write: clear them first
+    HIP_TRY(hipMemsetAsync(c->d_plain_partials, 0, need * sizeof(float), c->stream));
+    {
+        TimedLaunch t(c, 2);
+        HIP_TRY(launch_energy_plain(c->params(), replica, s.n_beads, block, c->d_plain_partials, c->stream));
+    }
+    std::vector<float> h(need);
+    HIP_TRY(hipMemcpyAsync(h.data(), c->d_plain_partials, need * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (size_t i = 0; i < need; ++i) sum += (double)h[i];
+    *out = sum;
+    return REMC_OK;
+}
+
+int remc_energy_all(remc_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    const int R = (int)c->reps.size();
+    {
+        TimedLaunch t(c, 1);
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(out, c->d_out, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_refresh_energies(remc_ctx *c)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    const int R = (int)c->reps.size();
+    {
+        TimedLaunch t(c, 1);
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, c->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_energy_noncrowder_all(remc_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    const int R = (int)c->reps.size();
+    {
+        TimedLaunch t(c, 1);
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 1, 0, c->stream));
+    }
+    HIP_TRY(hipMemcpyAsync(out, c->d_out, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_mc_steps(remc_ctx *c, int nsteps)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
+    if (nsteps == 0) return REMC_OK;
+    int rc = ensure_mc_block(c);
+    if (rc != REMC_OK) return rc;
+    {
+        TimedLaunch t(c, 0);
+        HIP_TRY(launch_mc(c->params(), (int)c->reps.size(), nsteps, c->mc_block, c->stream));
+    }
+    return REMC_OK;
+}
+
+int remc_mc_steps_plain(remc_ctx *c, int replica, int nsteps, int block)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
+    const HostRep &h = c->reps[replica];
+    const HostSys &s = c->systems[h.sys];
+    const int N = s.n_beads, M = s.n_bodies;
+    const int bo = c->rep_bead_off[replica], mo = c->rep_body_off[replica];
+    hipStream_t st = c->stream;
+    // host mirror of the replica (Replica's own Residue/Molecule arrays)
+    std::vector<float> x(N), y(N), z(N), rx(N), ry(N), rz(N), cx(M), cy(M), cz(M);
+    std::vector<double> quat(4 * M);
+    double E;
+    unsigned long long step, cnt[3];
+    HIP_TRY(hipMemcpyAsync(x.data(), c->d_px + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(y.data(), c->d_py + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(z.data(), c->d_pz + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(rx.data(), c->d_rx + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(ry.data(), c->d_ry + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(rz.data(), c->d_rz + bo, 4 * N, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cx.data(), c->d_cx + mo, 4 * M, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cy.data(), c->d_cy + mo, 4 * M, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cz.data(), c->d_cz + mo, 4 * M, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(quat.data(), c->d_quat + 4 * mo, 32 * M, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&E, c->d_E + replica, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(&step, c->d_step + replica, 8, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipMemcpyAsync(cnt, c->d_cnt + 3 * replica, 24, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    const double ch = std::cos(h.rstep / 2.0), sh = std::sin(h.rstep / 2.0);
+    const float box = s.box > 0.f ? s.box : 0.f;
+    const uint32_t k0 = (uint32_t)c->seed, k1 = (uint32_t)(c->seed >> 32);
+    std::vector<float> nx(s.max_body), ny(s.max_body), nz(s.max_body);
+    for (int it = 0; it < nsteps; ++it, ++step) {
+        remc_draws d = remc_step_draws(step, (uint32_t)replica, k0, k1, M);
+        const int m0 = s.body_start[d.body], m1 = s.body_start[d.body + 1], nm = m1 - m0;
+        double qmv[4], coef[9];
+        float ncx = cx[d.body], ncy = cy[d.body], ncz = cz[d.body];
+        if (d.rotate) {
+            remc_move_quat(d.a, d.b, d.c, ch, sh, qmv);
+            remc_qcoeffs(qmv, coef);
+            for (int k = 0; k < nm; ++k) {
+                float a, b, cc;
+                remc_qrotate(coef, rx[m0 + k], ry[m0 + k], rz[m0 + k], &a, &b, &cc);
+                nx[k] = a + ncx; ny[k] = b + ncy; nz[k] = cc + ncz;
+            }
+        } else {
+            remc_move_centre(d.a, d.b, d.c, h.tstep, box, cx[d.body], cy[d.body], cz[d.body], &ncx, &ncy, &ncz);
+            for (int k = 0; k < nm; ++k) {
+                nx[k] = rx[m0 + k] + ncx; ny[k] = ry[m0 + k] + ncy; nz[k] = rz[m0 + k] + ncz;
+            }
+        }
+        // MoleculeDataToDevice (Replica.cpp:424)
+        HIP_TRY(hipMemcpyAsync(c->d_px + bo + m0, nx.data(), 4 * nm, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(c->d_py + bo + m0, ny.data(), 4 * nm, hipMemcpyHostToDevice, st));
+        HIP_TRY(hipMemcpyAsync(c->d_pz + bo + m0, nz.data(), 4 * nm, hipMemcpyHostToDevice, st));
+        double e_new;
+        int rc = remc_energy_plain(c, replica, block, &e_new);
+        if (rc != REMC_OK) return rc;
+        const double delta = e_new - E;
+        const int dec = remc_metropolis(delta, h.T, d.u);
+        if (dec) {
+            for (int k = 0; k < nm; ++k) {
+                x[m0 + k] = nx[k]; y[m0 + k] = ny[k]; z[m0 + k] = nz[k];
+            }
+            if (d.rotate) {
+                for (int k = 0; k < nm; ++k) {
+                    float a, b, cc;
+                    remc_qrotate(coef, rx[m0 + k], ry[m0 + k], rz[m0 + k], &a, &b, &cc);
+                    rx[m0 + k] = a; ry[m0 + k] = b; rz[m0 + k] = cc;
+                }
+                double qo[4] = {quat[4 * d.body], quat[4 * d.body + 1], quat[4 * d.body + 2], quat[4 * d.body + 3]};
+                remc_qmul(qmv, qo, &quat[4 * d.body]);
+            } else {
+                cx[d.body] = ncx; cy[d.body] = ncy; cz[d.body] = ncz;
+            }
+            E = e_new;
+            cnt[dec == 1 ? 0 : 1]++;
+        } else {
+            cnt[2]++;
+            // undo + MoleculeDataToDevice again (Replica.cpp:468-471)
+            HIP_TRY(hipMemcpyAsync(c->d_px + bo + m0, x.data() + m0, 4 * nm, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(c->d_py + bo + m0, y.data() + m0, 4 * nm, hipMemcpyHostToDevice, st));
+            HIP_TRY(hipMemcpyAsync(c->d_pz + bo + m0, z.data() + m0, 4 * nm, hipMemcpyHostToDevice, st));
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_rx + bo, rx.data(), 4 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_ry + bo, ry.data(), 4 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_rz + bo, rz.data(), 4 * N, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_cx + mo, cx.data(), 4 * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_cy + mo, cy.data(), 4 * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_cz + mo, cz.data(), 4 * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_quat + 4 * mo, quat.data(), 32 * M, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_E + replica, &E, 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_step + replica, &step, 8, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(c->d_cnt + 3 * replica, cnt, 24, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return REMC_OK;
+}
+
+int remc_exchange(remc_ctx *c, int *n_swaps)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    const int S = (int)c->systems.size();
+    std::vector<unsigned long long> before(S), after(S);
+    if (n_swaps)
+        HIP_TRY(hipMemcpyAsync(before.data(), c->d_exswaps, 8 * S, hipMemcpyDeviceToHost, c->stream));
+    {
+        TimedLaunch t(c, 3);
+        HIP_TRY(launch_exchange(c->params(), S, c->stream));
+    }
+    if (n_swaps) {
+        HIP_TRY(hipMemcpyAsync(after.data(), c->d_exswaps, 8 * S, hipMemcpyDeviceToHost, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        unsigned long long n = 0;
+        for (int i = 0; i < S; ++i) n += after[i] - before[i];
+        *n_swaps = (int)n;
+    }
+    return REMC_OK;
+}
+
+int remc_run(remc_ctx *c, int rounds, int steps_per_round)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (rounds < 0 || steps_per_round < 0) return fail(REMC_ERR_ARG, "negative rounds or steps");
+    if (rounds == 0) return REMC_OK;
+    int rc = ensure_mc_block(c);
+    if (rc != REMC_OK) return rc;
+    if (!c->graph || c->graph_steps != steps_per_round) {
+        destroy_graph(c);
+        KParams p = c->params();
+        hipGraph_t g = nullptr;
+        HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e1 = launch_mc(p, (int)c->reps.size(), steps_per_round, c->mc_block, c->stream);
+        hipError_t e2 = launch_exchange(p, (int)c->systems.size(), c->stream);
+        hipError_t e3 = hipStreamEndCapture(c->stream, &g);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return fail(REMC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
+        }
+        hipError_t e4 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e4 != hipSuccess) {
+            c->graph = nullptr;
+            return fail(REMC_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e4));
+        }
+        c->graph_steps = steps_per_round;
+    }
+    for (int i = 0; i < rounds; ++i) {
+        TimedLaunch t(c, 0);
+        HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+    }
+    return REMC_OK;
+}
+
+int remc_set_recording(remc_ctx *c, int max_steps)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (max_steps < 0) return fail(REMC_ERR_ARG, "negative recording length");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->d_rec) hipFree(c->d_rec);
+    c->d_rec = nullptr;
+    c->rec_max = 0;
+    destroy_graph(c);
+    if (max_steps == 0) return REMC_OK;
+    const int R = (int)c->reps.size();
+    HIP_TRY(hipMalloc(&c->d_rec, sizeof(remc_step_record) * (size_t)R * max_steps));
+    HIP_TRY(hipMemsetAsync(c->d_rec, 0xff, sizeof(remc_step_record) * (size_t)R * max_steps, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_rec_start, c->d_step, sizeof(unsigned long long) * R, hipMemcpyDeviceToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->rec_max = max_steps;
+    return REMC_OK;
+}
+
+int remc_get_records(remc_ctx *c, int replica, remc_step_record *out, int n)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    if (!out || n < 0 || n > c->rec_max) return fail(REMC_ERR_ARG, "bad record request");
+    HIP_TRY(hipMemcpyAsync(out, c->d_rec + (size_t)replica * c->rec_max, sizeof(remc_step_record) * n,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_get_energies(remc_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    HIP_TRY(hipMemcpyAsync(out, c->d_E, sizeof(double) * c->reps.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_get_temperatures(remc_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    HIP_TRY(hipMemcpyAsync(out, c->d_T, sizeof(double) * c->reps.size(), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_get_counters(remc_ctx *c, remc_counters *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    const int R = (int)c->reps.size();
+    std::vector<unsigned long long> cnt(3 * R), st(R);
+    HIP_TRY(hipMemcpyAsync(cnt.data(), c->d_cnt, 24 * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(st.data(), c->d_step, 8 * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < R; ++r) {
+        out[r].accept = cnt[3 * r];
+        out[r].accept_boltz = cnt[3 * r + 1];
+        out[r].reject = cnt[3 * r + 2];
+        out[r].steps = st[r];
+    }
+    return REMC_OK;
+}
+
+int remc_get_ladder(remc_ctx *c, int sys_id, int *slot_replica)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size() || !slot_replica) return fail(REMC_ERR_ARG, "bad ladder request");
+    int off = 0;
+    for (int i = 0; i < sys_id; ++i) off += (int)c->systems[i].replicas.size();
+    HIP_TRY(hipMemcpyAsync(slot_replica, c->d_slot + off, sizeof(int) * c->systems[sys_id].replicas.size(),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_get_state(remc_ctx *c, int replica, float *x, float *y, float *z, float *rx, float *ry, float *rz,
+                   float *cx, float *cy, float *cz, double *quat)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    const HostSys &s = c->systems[c->reps[replica].sys];
+    const size_t nb = s.n_beads, nm = s.n_bodies;
+    const int bo = c->rep_bead_off[replica], mo = c->rep_body_off[replica];
+    hipStream_t st = c->stream;
+    float *dst_b[6] = {x, y, z, rx, ry, rz};
+    float *src_b[6] = {c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz};
+    for (int i = 0; i < 6; ++i)
+        if (dst_b[i]) HIP_TRY(hipMemcpyAsync(dst_b[i], src_b[i] + bo, 4 * nb, hipMemcpyDeviceToHost, st));
+    float *dst_m[3] = {cx, cy, cz};
+    float *src_m[3] = {c->d_cx, c->d_cy, c->d_cz};
+    for (int i = 0; i < 3; ++i)
+        if (dst_m[i]) HIP_TRY(hipMemcpyAsync(dst_m[i], src_m[i] + mo, 4 * nm, hipMemcpyDeviceToHost, st));
+    if (quat) HIP_TRY(hipMemcpyAsync(quat, c->d_quat + 4 * (size_t)mo, 32 * nm, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return REMC_OK;
+}
+
+int remc_set_state(remc_ctx *c, int replica, const float *x, const float *y, const float *z, const float *rx,
+                   const float *ry, const float *rz, const float *cx, const float *cy, const float *cz,
+                   const double *quat)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    const HostSys &s = c->systems[c->reps[replica].sys];
+    const size_t nb = s.n_beads, nm = s.n_bodies;
+    const int bo = c->rep_bead_off[replica], mo = c->rep_body_off[replica];
+    hipStream_t st = c->stream;
+    const float *src_b[6] = {x, y, z, rx, ry, rz};
+    const float *src_c[3] = {cx, cy, cz};
+    for (int i = 0; i < 6; ++i)
+        if (src_b[i])
+            for (size_t k = 0; k < nb; ++k)
+                if (!std::isfinite(src_b[i][k])) return fail(REMC_ERR_ARG, "non-finite bead coordinate");
+    for (int i = 0; i < 3; ++i)
+        if (src_c[i])
+            for (size_t k = 0; k < nm; ++k)
+                if (!std::isfinite(src_c[i][k])) return fail(REMC_ERR_ARG, "non-finite body centre");
+    if (quat)
+        for (size_t k = 0; k < 4 * nm; ++k)
+            if (!std::isfinite(quat[k])) return fail(REMC_ERR_ARG, "non-finite rotation");
+    float *dst_b[6] = {c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz};
+    for (int i = 0; i < 6; ++i)
+        if (src_b[i]) HIP_TRY(hipMemcpyAsync(dst_b[i] + bo, src_b[i], 4 * nb, hipMemcpyHostToDevice, st));
+    const float *src_m[3] = {cx, cy, cz};
+    float *dst_m[3] = {c->d_cx, c->d_cy, c->d_cz};
+    for (int i = 0; i < 3; ++i)
+        if (src_m[i]) HIP_TRY(hipMemcpyAsync(dst_m[i] + mo, src_m[i], 4 * nm, hipMemcpyHostToDevice, st));
+    if (quat) HIP_TRY(hipMemcpyAsync(c->d_quat + 4 * (size_t)mo, quat, 32 * nm, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return REMC_OK;
+}
+
+int remc_adjoint(remc_ctx *c, int replica, double *force_torque, double *dparams)
+{
+    CHECK_CTX(c);
+    (void)replica; (void)force_torque; (void)dparams;
+    return fail(REMC_ERR_STATE, "adjoint kernel not built in this library version");
+}
+
+int remc_timing_enable(remc_ctx *c, int enable)
+{
+    CHECK_CTX(c);
+    int rc = collect_timing(c);
+    if (rc != REMC_OK) return rc;
+    for (int i = 0; i < 5; ++i) {
+        c->t_total[i] = 0.0;
+        c->t_count[i] = 0;
+    }
+    c->timing = enable != 0;
+    return REMC_OK;
+}
+
+int remc_timing_get(remc_ctx *c, int kernel_class, double *total_ms, int64_t *launches)
+{
+    CHECK_CTX(c);
+    if (kernel_class < 0 || kernel_class > 4) return fail(REMC_ERR_ARG, "kernel class out of range");
+    int rc = collect_timing(c);
+    if (rc != REMC_OK) return rc;
+    if (total_ms) *total_ms = c->t_total[kernel_class];
+    if (launches) *launches = c->t_count[kernel_class];
+    return REMC_OK;
+}
+
+int remc_synchronize(remc_ctx *c)
+{
+    CHECK_CTX(c);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_set_mc_block(remc_ctx *c, int block)
+{
+    CHECK_CTX(c);
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+        return fail(REMC_ERR_ARG, "MC block must be 0, 256, 512 or 1024");
+    c->mc_block = block;
+    destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_get_mc_block(remc_ctx *c, int *block)
+{
+    CHECK_CTX(c);
+    if (!block) return fail(REMC_ERR_ARG, "null output");
+    *block = c->mc_block;
+    return REMC_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,94 @@
+// remc_internal.h -- device data layout shared by the kernels and the C-ABI
+// implementation.  Everything lives in HBM as structure-of-arrays:
+//
+//   static, per bead (all systems concatenated, index sys.bead_off + b):
+//     tc   int    type | crowder<<8
+//     q    float  charge
+//     sig  float  vdW radius
+//     body int    body index local to the system
+//   static, per body (index sys.body_off + m): bstart, bend (local bead
+//     indices), bcrow (crowder flag)
+//   dynamic, per replica bead (index rep.bead_off + b): px,py,pz positions,
+//     rx,ry,rz positions relative to the body centre
+//   dynamic, per replica body (index rep.body_off + m): cx,cy,cz centre,
+//     quat[4*(...)] rotation (w,x,y,z) in FP64
+//   dynamic, per replica: E (FP64, kcal/mol), T (FP64), step (RNG counter),
+//     cnt[3] accept/accept_boltz/reject
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "remc.h"
+
+namespace remc {
+
+struct DevSys {
+    int bead_off, n_beads, body_off, n_bodies;
+    int max_body, rep_first, n_rep, slot_off;
+    float box, inv_box;
+    int pad0, pad1;
+};
+
+struct DevRep {
+    int sys, bead_off, body_off;
+    uint32_t rng_id;
+    float tstep, pad;
+    double rot_ch, rot_sh;
+};
+
+// prescaled force field: etab[i*20+j] = lambda*(table-e0)*lj_cf*kbt_cf
+struct DevFF {
+    float c_crow;    // lj_cf*kbt_cf (crowder term coefficient)
+    float crow_s2;   // crowder sigma^2
+    float crow_cut2; // crowder cutoff^2
+    float c_dh;      // dh_cf*kbt_cf
+    float k_dh;      // -log2(e)/xi, exp(-r/xi) = exp2(r*k_dh)
+    float eps_r;     // 1e-38 added to r
+    float pad0, pad1;
+};
+
+struct KParams {
+    // static
+    const int *tc;
+    const float *q;
+    const float *sig;
+    const int *body;
+    const int *bstart;
+    const int *bend;
+    const int *bcrow;
+    const DevSys *sys;
+    const DevRep *rep;
+    const float *etab;
+    DevFF ff;
+    // dynamic
+    float *px, *py, *pz, *rx, *ry, *rz, *cx, *cy, *cz;
+    double *quat;
+    double *E, *T;
+    unsigned long long *step;
+    unsigned long long *cnt;
+    // exchange ladder
+    int *slot_rep;          // per system slots: slot_rep[sys.slot_off + k]
+    double *ladder;         // ladder[sys.slot_off + k]
+    unsigned long long *ex_round;  // per system
+    int *ex_offset;         // per system
+    unsigned long long *ex_swaps;  // per system cumulative
+    // recording
+    remc_step_record *rec;
+    unsigned long long *rec_start;
+    int rec_max;
+    uint32_t key0, key1;
+};
+
+// launchers (remc_kernels.hip)
+hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s);
+hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
+                              int noncrowder_only, int write_E, hipStream_t s);
+hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
+hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
+hipError_t launch_adjoint(const KParams &p, int replica, int n_beads, int n_bodies, double *bead_grad,
+                          double *param_grad, hipStream_t s);
+
+constexpr int FULL_TILE = 256;
+
+} // namespace remc

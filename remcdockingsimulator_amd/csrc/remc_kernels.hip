@@ -1,0 +1,478 @@
+// remc_kernels.hip -- gfx950 kernels of the REMC hot path.
+//
+//   k_mc<BLOCK>        batched persistent MC: one workgroup per replica runs
+//                      nsteps steps (proposal, dE of the moved body against all
+//                      other beads, FP64 Metropolis, commit) without leaving
+//                      the GPU.  Replaces Replica::MCSearch's per-step
+//                      MoleculeDataToDevice/EonDevice round trip
+//                      (Replica.cpp:368-478).
+//   k_energy_full      batched full energy, every replica in one launch,
+//                      256-bead tiles, half-window tile schedule (each
+//                      unordered tile pair once), Kahan FP32 per thread,
+//                      FP64 per tile; k_sum_partials adds tiles in fixed order.
+//   k_energy_plain     the reference-shaped kernel (E_TiledKernel,
+//                      CudaFunctions.cu:455-685): (G,G) grid of B-bead tiles,
+//                      lower triangle culled, FP32 partial per tile, diagonal
+//                      tile halved.  The host adds the partials in FP64.
+//   k_exchange         temperature-swap replica exchange per system
+//                      (main.cpp:1427-1453).
+//
+// Pair term (Replica.cpp:584-600,694-733; CudaFunctions.cu:430-452,536-649):
+// minimum image, r = |d|; crowder pairs c*(6/r)^6 for r < 15; else
+// E_ij*(...) contact term with the repulsive flip plus Debye-Hueckel.  The
+// table etab already holds lambda*(T_ij - e0)*lj_cf*kbt_cf, the DH
+// coefficient dh_cf*kbt_cf, so a pair's value is directly kcal/mol.
+// x^6 is formed by multiplication; exp(-r/xi) = exp2(r*k_dh) with the
+// hardware exp2; 1/r from the hardware reciprocal square root.
+#include "remc_internal.h"
+#include "remc_math.h"
+
+namespace remc {
+
+__device__ __forceinline__ float min_image(float d, float box, float inv_box)
+{
+    return __builtin_fmaf(-box, __builtin_rintf(d * inv_box), d);
+}
+
+// Energy of one non-crowder pair from squared distance.
+__device__ __forceinline__ float pair_lj_dh(float r2, float eij, float sij2, float qq, const DevFF &ff)
+{
+    r2 = fmaxf(r2, 1e-36f);
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float r = r2 * rinv;
+    float s2 = sij2 * (rinv * rinv);
+    float x6 = s2 * s2 * s2;
+    float x4 = 4.0f * x6;
+    float base = __builtin_fmaf(x4, x6, -x4);           // 4 x6 (x6 - 1)
+    bool flip = (eij > 0.0f) & (x6 > 0.5f);             // r < 2^(1/6) sigma
+    float lj = flip ? eij * (base + 2.0f) : -eij * base;
+    float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+    // coincident beads overflow to inf (or 0*inf); cap so compensated sums stay finite
+    return fminf(lj + dh, 1e30f);
+}
+
+__device__ __forceinline__ float pair_crowder(float r2, const DevFF &ff)
+{
+    r2 = fmaxf(r2, 1e-36f);
+    float t = ff.crow_s2 * __builtin_amdgcn_rcpf(r2);
+    float t3 = t * t * t;
+    return r2 < ff.crow_cut2 ? fminf(ff.c_crow * t3, 1e30f) : 0.0f;
+}
+
+// Kahan step, kept as written (no reassociation is enabled for this file).
+__device__ __forceinline__ void kahan_add(float &s, float &c, float v)
+{
+    float y = v - c;
+    float t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// --------------------------------------------------------------------------
+// batched persistent MC
+// --------------------------------------------------------------------------
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
+    __shared__ float4 s_old[REMC_MAX_BODY_BEADS]; // x,y,z,q   current positions of the moved body
+    __shared__ float4 s_new[REMC_MAX_BODY_BEADS]; // x,y,z,sig proposed positions
+    __shared__ int s_tc[REMC_MAX_BODY_BEADS];
+    __shared__ double s_red[NW];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int r = blockIdx.x;
+    const DevRep R = p.rep[r];
+    const DevSys S = p.sys[R.sys];
+    const DevFF ff = p.ff;
+
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
+
+    double E = p.E[r];
+    const double T = p.T[r];
+    unsigned long long step = p.step[r];
+    unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
+    const unsigned long long rec_start = p.rec ? p.rec_start[r] : 0ull;
+
+    const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    const int *tcs = p.tc + S.bead_off;
+    const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
+    const int N = S.n_beads;
+    __syncthreads();
+
+    for (int it = 0; it < nsteps; ++it, ++step) {
+        // draws and move parameters: every lane computes the same values
+        remc_draws d = remc_step_draws(step, R.rng_id, p.key0, p.key1, S.n_bodies);
+        const int gb = S.body_off + d.body;
+        const int m0 = p.bstart[gb], m1 = p.bend[gb], nm = m1 - m0;
+        const bool cm = p.bcrow[gb] != 0;
+        const int rb = R.body_off + d.body;
+        const float cx = p.cx[rb], cy = p.cy[rb], cz = p.cz[rb];
+        double coef[9];
+        double qmv[4];
+        float ncx = cx, ncy = cy, ncz = cz;
+        if (d.rotate) {
+            remc_move_quat(d.a, d.b, d.c, R.rot_ch, R.rot_sh, qmv);
+            remc_qcoeffs(qmv, coef);
+        } else {
+            remc_move_centre(d.a, d.b, d.c, R.tstep, S.box, cx, cy, cz, &ncx, &ncy, &ncz);
+        }
+        for (int k = tid; k < nm; k += BLOCK) {
+            const int b = m0 + k;
+            float rx = p.rx[R.bead_off + b], ry = p.ry[R.bead_off + b], rz = p.rz[R.bead_off + b];
+            float nrx = rx, nry = ry, nrz = rz;
+            if (d.rotate) remc_qrotate(coef, rx, ry, rz, &nrx, &nry, &nrz);
+            s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b]);
+            s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
+            s_tc[k] = tcs[b];
+        }
+        __syncthreads();
+
+        // dE = sum_{i in body} sum_{j not in body} e(new_i, j) - e(old_i, j)
+        float acc = 0.0f, comp = 0.0f;
+        for (int j = tid; j < N; j += BLOCK) {
+            if (j >= m0 && j < m1) continue;
+            const float xj = gx[j], yj = gy[j], zj = gz[j];
+            const int tcj = tcs[j];
+            const bool cj = (tcj >> 8) != 0;
+            if (cm || cj) {
+                for (int k = 0; k < nm; ++k) {
+                    const float4 o = s_old[k], n = s_new[k];
+                    float dx = min_image(xj - o.x, S.box, S.inv_box);
+                    float dy = min_image(yj - o.y, S.box, S.inv_box);
+                    float dz = min_image(zj - o.z, S.box, S.inv_box);
+                    float eo = pair_crowder(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), ff);
+                    dx = min_image(xj - n.x, S.box, S.inv_box);
+                    dy = min_image(yj - n.y, S.box, S.inv_box);
+                    dz = min_image(zj - n.z, S.box, S.inv_box);
+                    float en = pair_crowder(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), ff);
+                    kahan_add(acc, comp, en - eo);
+                }
+            } else {
+                const float qj = qs[j] * ff.c_dh, sj = sgs[j];
+                const float *erow = s_etab + (tcj & 0xff);
+                for (int k = 0; k < nm; ++k) {
+                    const float4 o = s_old[k], n = s_new[k];
+                    const int ti = s_tc[k] & 0xff;
+                    const float eij = erow[ti * REMC_NTYPES];
+                    const float sij = 0.5f * (n.w + sj);
+                    const float sij2 = sij * sij;
+                    const float qq = o.w * qj;
+                    float dx = min_image(xj - o.x, S.box, S.inv_box);
+                    float dy = min_image(yj - o.y, S.box, S.inv_box);
+                    float dz = min_image(zj - o.z, S.box, S.inv_box);
+                    float eo = pair_lj_dh(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), eij, sij2, qq, ff);
+                    dx = min_image(xj - n.x, S.box, S.inv_box);
+                    dy = min_image(yj - n.y, S.box, S.inv_box);
+                    dz = min_image(zj - n.z, S.box, S.inv_box);
+                    float en = pair_lj_dh(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), eij, sij2, qq, ff);
+                    kahan_add(acc, comp, en - eo);
+                }
+            }
+        }
+        double v = wave_sum((double)acc - (double)comp);
+        if (lane == 0) s_red[wave] = v;
+        __syncthreads();
+        double dE = 0.0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) dE += s_red[w];
+
+        const int decision = remc_metropolis(dE, T, d.u);
+        if (decision) {
+            for (int k = tid; k < nm; k += BLOCK) {
+                const int b = m0 + k;
+                const float4 n = s_new[k];
+                p.px[R.bead_off + b] = n.x;
+                p.py[R.bead_off + b] = n.y;
+                p.pz[R.bead_off + b] = n.z;
+                if (d.rotate) {
+                    float rx = p.rx[R.bead_off + b], ry = p.ry[R.bead_off + b], rz = p.rz[R.bead_off + b];
+                    float nrx, nry, nrz;
+                    remc_qrotate(coef, rx, ry, rz, &nrx, &nry, &nrz);
+                    p.rx[R.bead_off + b] = nrx;
+                    p.ry[R.bead_off + b] = nry;
+                    p.rz[R.bead_off + b] = nrz;
+                }
+            }
+            if (tid == 0) {
+                if (d.rotate) {
+                    double qo[4] = {p.quat[4 * rb + 0], p.quat[4 * rb + 1], p.quat[4 * rb + 2], p.quat[4 * rb + 3]};
+                    double qn[4];
+                    remc_qmul(qmv, qo, qn);
+                    p.quat[4 * rb + 0] = qn[0];
+                    p.quat[4 * rb + 1] = qn[1];
+                    p.quat[4 * rb + 2] = qn[2];
+                    p.quat[4 * rb + 3] = qn[3];
+                } else {
+                    p.cx[rb] = ncx;
+                    p.cy[rb] = ncy;
+                    p.cz[rb] = ncz;
+                }
+            }
+            E += dE;
+            if (decision == 1) ++n_acc; else ++n_accb;
+        } else {
+            ++n_rej;
+        }
+        if (p.rec && tid == 0) {
+            unsigned long long idx = step - rec_start;
+            if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
+                remc_step_record rr;
+                rr.body = d.body;
+                rr.move = d.rotate;
+                rr.accepted = decision;
+                rr.pad = 0;
+                rr.delta = dE;
+                p.rec[(size_t)r * p.rec_max + idx] = rr;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.E[r] = E;
+        p.step[r] = step;
+        p.cnt[3 * r + 0] += n_acc;
+        p.cnt[3 * r + 1] += n_accb;
+        p.cnt[3 * r + 2] += n_rej;
+    }
+}
+
+hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s)
+{
+    if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
+    switch (block) {
+    case 256: hipLaunchKernelGGL(k_mc<256>, dim3(n_rep), dim3(256), 0, s, p, nsteps); break;
+    case 512: hipLaunchKernelGGL(k_mc<512>, dim3(n_rep), dim3(512), 0, s, p, nsteps); break;
+    case 1024: hipLaunchKernelGGL(k_mc<1024>, dim3(n_rep), dim3(1024), 0, s, p, nsteps); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// batched full energy
+// --------------------------------------------------------------------------
+__global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int max_tiles, double *partials, int noncrowder_only)
+{
+    __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
+    __shared__ float s_x[FULL_TILE], s_y[FULL_TILE], s_z[FULL_TILE], s_q[FULL_TILE], s_s[FULL_TILE];
+    __shared__ int s_tc[FULL_TILE], s_body[FULL_TILE];
+    __shared__ double s_red[FULL_TILE / 64];
+
+    const int tid = threadIdx.x;
+    const int a = blockIdx.x, r = blockIdx.y;
+    const DevRep R = p.rep[r];
+    const DevSys S = p.sys[R.sys];
+    const int N = S.n_beads;
+    const int G = (N + FULL_TILE - 1) / FULL_TILE;
+    if (a >= G) return; // whole workgroup leaves before any barrier
+    const DevFF ff = p.ff;
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += FULL_TILE) s_etab[k] = p.etab[k];
+
+    const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
+    const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
+
+    const int i = a * FULL_TILE + tid;
+    const bool vi = i < N;
+    const float xi = vi ? gx[i] : 0.f, yi = vi ? gy[i] : 0.f, zi = vi ? gz[i] : 0.f;
+    const int tci = vi ? tcs[i] : 0, bi = vi ? bodies[i] : -1;
+    const float qi = vi ? qs[i] * ff.c_dh : 0.f, si = vi ? sgs[i] : 0.f;
+    const bool ci = (tci >> 8) != 0;
+    const float *erow = s_etab + (tci & 0xff) * REMC_NTYPES;
+
+    float acc = 0.0f, comp = 0.0f;
+    const int h = G / 2;
+    for (int bo = 0; bo <= h; ++bo) {
+        const int b = (a + bo) % G;
+        if ((G % 2 == 0) && bo == h && bo > 0 && a >= h) break;
+        __syncthreads();
+        const int j0 = b * FULL_TILE;
+        const int jt = j0 + tid;
+        if (jt < N) {
+            s_x[tid] = gx[jt]; s_y[tid] = gy[jt]; s_z[tid] = gz[jt];
+            s_q[tid] = qs[jt]; s_s[tid] = sgs[jt]; s_tc[tid] = tcs[jt]; s_body[tid] = bodies[jt];
+        } else {
+            s_body[tid] = -2;
+        }
+        __syncthreads();
+        if (!vi) continue;
+        const int nj = min(FULL_TILE, N - j0);
+        const int kstart = (bo == 0) ? tid + 1 : 0;
+        for (int k = kstart; k < nj; ++k) {
+            if (s_body[k] == bi) continue;
+            const int tcj = s_tc[k];
+            const bool cj = (tcj >> 8) != 0;
+            if (noncrowder_only && (ci || cj)) continue;
+            float dx = min_image(s_x[k] - xi, S.box, S.inv_box);
+            float dy = min_image(s_y[k] - yi, S.box, S.inv_box);
+            float dz = min_image(s_z[k] - zi, S.box, S.inv_box);
+            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            float e;
+            if (ci || cj) {
+                e = pair_crowder(r2, ff);
+            } else {
+                const float sij = 0.5f * (si + s_s[k]);
+                e = pair_lj_dh(r2, erow[tcj & 0xff], sij * sij, qi * s_q[k], ff);
+            }
+            kahan_add(acc, comp, e);
+        }
+    }
+    double v = wave_sum((double)acc - (double)comp);
+    if ((tid & 63) == 0) s_red[tid >> 6] = v;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < FULL_TILE / 64; ++w) t += s_red[w];
+        partials[(size_t)r * max_tiles + a] = t;
+    }
+}
+
+__global__ void k_sum_partials(KParams p, int n_rep, int max_tiles, const double *partials, double *out, int write_E)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rep) return;
+    const DevSys S = p.sys[p.rep[r].sys];
+    const int G = (S.n_beads + FULL_TILE - 1) / FULL_TILE;
+    double t = 0.0;
+    for (int a = 0; a < G; ++a) t += partials[(size_t)r * max_tiles + a];
+    if (out) out[r] = t;
+    if (write_E) p.E[r] = t;
+}
+
+hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
+                              int noncrowder_only, int write_E, hipStream_t s)
+{
+    if (n_rep <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_energy_full, dim3(max_tiles, n_rep), dim3(FULL_TILE), 0, s, p, max_tiles, partials,
+                       noncrowder_only);
+    hipLaunchKernelGGL(k_sum_partials, dim3((n_rep + 63) / 64), dim3(64), 0, s, p, n_rep, max_tiles, partials, out,
+                       write_E);
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// reference-shaped plain kernel (one replica, FP32 tiles, host FP64 sum)
+// --------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(B) void k_energy_plain(KParams p, int replica, float *partials)
+{
+    __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
+    __shared__ float s_x[B], s_y[B], s_z[B], s_q[B], s_s[B];
+    __shared__ int s_tc[B], s_body[B];
+    __shared__ float s_res[B];
+    const int tx = threadIdx.x;
+    const int bx = blockIdx.x, by = blockIdx.y;
+    const int G = gridDim.x;
+    if (bx < by) return; // lower triangle culled, whole block (CudaFunctions.cu:483-486)
+    const DevRep R = p.rep[replica];
+    const DevSys S = p.sys[R.sys];
+    const DevFF ff = p.ff;
+    const int N = S.n_beads;
+    for (int k = tx; k < REMC_NTYPES * REMC_NTYPES; k += B) s_etab[k] = p.etab[k];
+    const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
+    const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
+
+    const int jt = bx * B + tx;
+    if (jt < N) {
+        s_x[tx] = gx[jt]; s_y[tx] = gy[jt]; s_z[tx] = gz[jt];
+        s_q[tx] = qs[jt]; s_s[tx] = sgs[jt]; s_tc[tx] = tcs[jt]; s_body[tx] = bodies[jt];
+    } else {
+        s_body[tx] = -2; // padding
+    }
+    __syncthreads();
+    const int i = by * B + tx;
+    float sum = 0.0f;
+    if (i < N) {
+        const float xi = gx[i], yi = gy[i], zi = gz[i];
+        const int tci = tcs[i], bi = bodies[i];
+        const float qi = qs[i] * ff.c_dh, si = sgs[i];
+        const bool ci = (tci >> 8) != 0;
+        for (int k = 0; k < B; ++k) {
+            if (s_body[k] == bi || s_body[k] < 0) continue; // same body or padding
+            const int tcj = s_tc[k];
+            const bool cj = (tcj >> 8) != 0;
+            float dx = min_image(s_x[k] - xi, S.box, S.inv_box);
+            float dy = min_image(s_y[k] - yi, S.box, S.inv_box);
+            float dz = min_image(s_z[k] - zi, S.box, S.inv_box);
+            float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+            if (ci || cj) {
+                sum += pair_crowder(r2, ff);
+            } else {
+                const float sij = 0.5f * (si + s_s[k]);
+                sum += pair_lj_dh(r2, s_etab[(tci & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, qi * s_q[k], ff);
+            }
+        }
+        if (bx == by) sum *= 0.5f; // diagonal tile counts both orders (CudaFunctions.cu:651-653)
+    }
+    s_res[tx] = sum;
+    __syncthreads();
+    for (int off = B / 2; off > 0; off >>= 1) {
+        if (tx < off) s_res[tx] += s_res[tx + off];
+        __syncthreads();
+    }
+    if (tx == 0) partials[bx * G + by] = s_res[0];
+}
+
+hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s)
+{
+    const int G = (n_beads + block - 1) / block;
+    switch (block) {
+    case 32: hipLaunchKernelGGL(k_energy_plain<32>, dim3(G, G), dim3(32), 0, s, p, replica, partials); break;
+    case 64: hipLaunchKernelGGL(k_energy_plain<64>, dim3(G, G), dim3(64), 0, s, p, replica, partials); break;
+    case 128: hipLaunchKernelGGL(k_energy_plain<128>, dim3(G, G), dim3(128), 0, s, p, replica, partials); break;
+    case 256: hipLaunchKernelGGL(k_energy_plain<256>, dim3(G, G), dim3(256), 0, s, p, replica, partials); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// --------------------------------------------------------------------------
+// replica exchange
+// --------------------------------------------------------------------------
+__global__ void k_exchange(KParams p)
+{
+    const int sidx = blockIdx.x;
+    const DevSys S = p.sys[sidx];
+    const int offset = p.ex_offset[sidx];
+    const unsigned long long round = p.ex_round[sidx];
+    int *slots = p.slot_rep + S.slot_off;
+    const double *lad = p.ladder + S.slot_off;
+    int nsw = 0;
+    for (int k = offset + 2 * (int)threadIdx.x; k < S.n_rep - 1; k += 2 * (int)blockDim.x) {
+        const int ri = slots[k], rj = slots[k + 1];
+        const double u = remc_exchange_u(round, (uint32_t)k, (uint32_t)sidx, p.key0, p.key1);
+        if (remc_exchange_accept(lad[k], lad[k + 1], p.E[ri], p.E[rj], u)) {
+            slots[k] = rj;
+            slots[k + 1] = ri;
+            p.T[ri] = lad[k + 1];
+            p.T[rj] = lad[k];
+            ++nsw;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) nsw += __shfl_xor(nsw, off, 64);
+    if (threadIdx.x == 0) {
+        p.ex_swaps[sidx] += (unsigned long long)nsw;
+        p.ex_round[sidx] = round + 1;
+        p.ex_offset[sidx] = 1 - offset;
+    }
+}
+
+hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
+{
+    if (n_sys <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_exchange, dim3(n_sys), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+} // namespace remc

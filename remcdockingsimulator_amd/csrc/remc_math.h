@@ -1,0 +1,220 @@
+// remc_math.h -- arithmetic shared by the HIP kernels and the library's host
+// code (the host-side reference protocol of remc_mc_steps_plain).  Every
+// function here must give bit-identical results on gfx950 and on the host, so
+// contraction is switched off and transcendental functions are self-written
+// from IEEE basic operations and fused multiply-adds.
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define REMC_HD __host__ __device__ inline
+#else
+#include <math.h>
+#include <string.h>
+#define REMC_HD static inline
+#endif
+
+#define REMC_R_GAS 8.314472 /* Rgas, definitions.h:16 */
+#define REMC_TAG_MC 0x4D430000u
+#define REMC_TAG_EX 0x58000000u
+
+#pragma clang fp contract(off)
+
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11): 10 rounds of the
+// multiply-xor bijection with Weyl-sequence key schedule.
+struct remc_u32x4 {
+    uint32_t v[4];
+};
+
+REMC_HD remc_u32x4 remc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1)
+{
+    for (int round = 0; round < 10; ++round) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        c1 = (uint32_t)p1;
+        c3 = (uint32_t)p0;
+        c0 = n0;
+        c2 = n2;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    remc_u32x4 o;
+    o.v[0] = c0; o.v[1] = c1; o.v[2] = c2; o.v[3] = c3;
+    return o;
+}
+
+// uniform [0,1) with 32-bit resolution (gsl_rng_uniform on mt19937 resolution)
+REMC_HD double remc_u01(uint32_t w) { return (double)w * (1.0 / 4294967296.0); }
+
+// The draws of MC step `step` of replica `rng_id`, in the reference's draw
+// order (Replica.cpp:386-451): molecule, move type, three vector components,
+// Metropolis uniform.  The Metropolis uniform is drawn at a fixed counter and
+// used only when dE >= 0, as the reference draws it only then (:451).
+struct remc_draws {
+    int body;
+    int rotate;
+    double a, b, c, u;
+};
+
+REMC_HD remc_draws remc_step_draws(uint64_t step, uint32_t rng_id, uint32_t k0, uint32_t k1, int n_bodies)
+{
+    remc_u32x4 w0 = remc_philox((uint32_t)step, (uint32_t)(step >> 32), rng_id, REMC_TAG_MC, k0, k1);
+    remc_u32x4 w1 = remc_philox((uint32_t)step, (uint32_t)(step >> 32), rng_id, REMC_TAG_MC + 1u, k0, k1);
+    remc_draws d;
+    d.body = (int)(((uint64_t)w0.v[0] * (uint64_t)n_bodies) >> 32);
+    d.rotate = remc_u01(w0.v[1]) < 0.5;
+    d.a = remc_u01(w0.v[2]) - 0.5;
+    d.b = 0.5 - remc_u01(w0.v[3]);
+    d.c = remc_u01(w1.v[0]) - 0.5;
+    d.u = remc_u01(w1.v[1]);
+    return d;
+}
+
+REMC_HD double remc_exchange_u(uint64_t round, uint32_t pair, uint32_t system_id, uint32_t k0, uint32_t k1)
+{
+    remc_u32x4 w = remc_philox((uint32_t)round, (uint32_t)(round >> 32), pair, REMC_TAG_EX + (system_id & 0xFFFFFFu), k0, k1);
+    return remc_u01(w.v[0]);
+}
+
+// exp(x) in FP64: Cody-Waite reduction x = k ln2 + r (fdlibm split of ln2),
+// Taylor series of exp(r) to degree 13 in fused Horner form, exact 2^k scale.
+REMC_HD double remc_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.0) return __builtin_huge_val();
+    if (x < -708.0) return 0.0;
+    const double log2e = 1.4426950408889634;
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    double k = rint(x * log2e);
+    double r = fma(-k, ln2_hi, x);
+    r = fma(-k, ln2_lo, r);
+    double p = 1.0 / 6227020800.0;
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    uint64_t bits = (uint64_t)((int64_t)((int)k + 1023)) << 52;
+#if defined(__HIP_DEVICE_COMPILE__)
+    double two_k = __longlong_as_double((long long)bits);
+#else
+    double two_k;
+    memcpy(&two_k, &bits, sizeof two_k);
+#endif
+    return p * two_k;
+}
+
+// fmodf for finite x, L > 0, computed exactly by subtracting the largest
+// power-of-two multiple of L (each step exact by Sterbenz's lemma).
+REMC_HD float remc_fmodf(float x, float L)
+{
+    float ax = fabsf(x);
+    if (!(ax >= L)) return x;           // also NaN
+    if (!(ax <= 3.4028234663852886e38f)) return x; // inf: no finite remainder, leave it
+    float r = ax;
+    while (r >= L) {
+        float d = L;
+        while (2.0f * d <= r) d = 2.0f * d;
+        r = r - d;
+    }
+    return x < 0.0f ? -r : r;
+}
+
+// Quaternion product a*b (Quaternion.cpp:35-47), (w,x,y,z).
+REMC_HD void remc_qmul(const double a[4], const double b[4], double o[4])
+{
+    double w = a[0] * b[0] - a[1] * b[1] - a[2] * b[2] - a[3] * b[3];
+    double x = a[0] * b[1] + a[1] * b[0] + a[2] * b[3] - a[3] * b[2];
+    double y = a[0] * b[2] - a[1] * b[3] + a[2] * b[0] + a[3] * b[1];
+    double z = a[0] * b[3] + a[1] * b[2] - a[2] * b[1] + a[3] * b[0];
+    o[0] = w; o[1] = x; o[2] = y; o[3] = z;
+}
+
+// Rotation of a float vector by a unit quaternion in FP64, rounded to float
+// (Quaternion::rotateVector, Quaternion.cpp:84-101).  The coefficients are
+// passed precomputed: m = {w2+x2-y2-z2, xy-wz, wy+xz, wz+xy, yz-wx,
+// w2-x2+y2-z2, xz-wy, yz+wx, w2-x2-y2+z2}.
+REMC_HD void remc_qcoeffs(const double q[4], double m[9])
+{
+    double w = q[0], x = q[1], y = q[2], z = q[3];
+    double w2 = w * w, x2 = x * x, y2 = y * y, z2 = z * z;
+    double wx = w * x, wy = w * y, wz = w * z, xy = x * y, xz = x * z, yz = y * z;
+    m[0] = w2 + x2 - y2 - z2;
+    m[1] = xy - wz;
+    m[2] = wy + xz;
+    m[3] = wz + xy;
+    m[4] = yz - wx;
+    m[5] = w2 - x2 + y2 - z2;
+    m[6] = xz - wy;
+    m[7] = yz + wx;
+    m[8] = w2 - x2 - y2 + z2;
+}
+
+REMC_HD void remc_qrotate(const double m[9], float vx_, float vy_, float vz_, float *ox, float *oy, float *oz)
+{
+    double vx = vx_, vy = vy_, vz = vz_;
+    *ox = (float)(vx * m[0] + (vy * m[1] + vz * m[2]) * 2.0);
+    *oy = (float)((vx * m[3] + vz * m[4]) * 2.0 + vy * m[5]);
+    *oz = (float)((vx * m[6] + vy * m[7]) * 2.0 + vz * m[8]);
+}
+
+// Rotation quaternion of a rotate move: axis normalize(a,b,c) in FP64
+// (Vector3double::normalizeInPlace), q = (cos, sin*axis) normalised
+// (Molecule::rotateQ, Molecule.cpp:290-296).
+REMC_HD void remc_move_quat(double a, double b, double c, double ch, double sh, double q[4])
+{
+    double mag = sqrt(a * a + b * b + c * c);
+    double ax = a / mag, ay = b / mag, az = c / mag;
+    q[0] = ch; q[1] = sh * ax; q[2] = sh * ay; q[3] = sh * az;
+    double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    q[0] = q[0] / qm; q[1] = q[1] / qm; q[2] = q[2] / qm; q[3] = q[3] / qm;
+}
+
+// New centre of a translate move (Replica::translate, Replica.cpp:333-364):
+// t = step * normalize((float)a,(float)b,(float)c) in FP32, wrapped with fmod.
+REMC_HD void remc_move_centre(double a, double b, double c, float step, float box, float cx, float cy, float cz,
+                              float *px, float *py, float *pz)
+{
+    float vx = (float)a, vy = (float)b, vz = (float)c;
+    float mag = sqrtf(vx * vx + vy * vy + vz * vz);
+    if (mag != 0.0f) { vx = vx / mag; vy = vy / mag; vz = vz / mag; }
+    float x = cx + vx * step, y = cy + vy * step, z = cz + vz * step;
+    if (box > 0.0f) {
+        x = remc_fmodf(x + box, box);
+        y = remc_fmodf(y + box, box);
+        z = remc_fmodf(z + box, box);
+    }
+    *px = x; *py = y; *pz = z;
+}
+
+// Metropolis test in FP64 (Replica.cpp:438-459): 1 = accept (dE < 0),
+// 2 = Boltzmann accept, 0 = reject.
+REMC_HD int remc_metropolis(double delta, double temperature, double u)
+{
+    if (delta < 0.0) return 1;
+    double beta = 4184.0 / (REMC_R_GAS * temperature);
+    return u < remc_exp(-delta * beta) ? 2 : 0;
+}
+
+// Exchange test of ladder rungs (Ti, Tj) holding energies (Ei, Ej)
+// (main.cpp:1440-1441).
+REMC_HD int remc_exchange_accept(double Ti, double Tj, double Ei, double Ej, double u)
+{
+    double delta = (1.0 / Ti - 1.0 / Tj) * (Ei - Ej) * (4184.0 / REMC_R_GAS);
+    return delta >= 0.0 || u < remc_exp(delta);
+}
+
+#pragma clang fp contract(fast)

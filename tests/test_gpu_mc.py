@@ -1,0 +1,168 @@
+"""GPU Monte Carlo and replica exchange through the C ABI against the oracle
+under identical Philox streams.
+
+Bars: bodies and move types identical at every step (RNG parity); per-step dE
+within 1e-4 kT of the FP64 oracle; accept/reject decisions identical, except
+at a step whose FP64 decision margin is inside the 1e-4 kT guard band (then
+the chains may diverge from that step on and the comparison stops there);
+positions bit-identical while decisions agree (the move arithmetic is shared
+IEEE code); tracked energies within 1e-5 relative of the oracle's full
+energy of the GPU's own final conformation."""
+import numpy as np
+import pytest
+
+from helpers import compare_decisions, engine_for, kT, say
+from remcdockingsimulator_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SEED = 0xC0FFEE
+
+
+def run_parity(oracle, cid, nsteps, block=None, replicas=None):
+    wl = synth.config(cid)
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED, block=block)
+    e.set_recording(nsteps)
+    e.mc_steps(nsteps)
+    e.synchronize()
+    o = oracle.OracleSystem(sa, wl.ff)
+    energies = e.energies()
+    full = e.energy_all()
+    reps = range(len(lad)) if replicas is None else replicas
+    diverged = 0
+    for r in reps:
+        say("config", cid, "replica", r, "block", e.mc_block())
+        grec = e.records(r)
+        orep = oracle.OracleReplica(o, lad[r], wl.translate_step, wl.rotate_step, r)
+        orec = orep.mc(seed=SEED, nsteps=nsteps, mode=0)
+        n_eq, div = compare_decisions(grec, orec, lad[r])
+        st = e.state(r)
+        if not div:
+            assert np.array_equal(st["pos"], orep.pos), r
+            assert np.array_equal(st["rel"], orep.rel), r
+            assert np.array_equal(st["center"], orep.center), r
+            assert np.array_equal(st["quat"], orep.quat), r
+            assert abs(energies[r] - orep.energy) < 1e-4 * kT(lad[r])
+        else:
+            diverged += 1
+        want = o.energy(st["pos"])
+        assert abs(full[r] - want) <= 1e-5 * abs(want) + 1e-7
+        assert abs(energies[r] - want) <= 1e-5 * abs(want) + 1e-4 * kT(lad[r])
+    e.close()
+    return diverged
+
+
+def test_mc_parity_config1(oracle):
+    assert run_parity(oracle, 1, 2000) <= 1
+
+
+def test_mc_parity_config2(oracle):
+    assert run_parity(oracle, 2, 1500) <= 1
+
+
+def test_mc_parity_config4(oracle):
+    assert run_parity(oracle, 4, 300, replicas=range(0, 32, 4)) <= 1
+
+
+@pytest.mark.parametrize("block", [256, 512, 1024])
+def test_mc_block_sizes_agree(oracle, block):
+    assert run_parity(oracle, 2, 400, block=block, replicas=[0, 7, 19]) <= 1
+
+
+def test_plain_protocol_matches_oracle(oracle):
+    """remc_mc_steps_plain: the reference's per-step host protocol with the
+    reference-shaped kernel takes the oracle's decisions."""
+    wl = synth.config(2)
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED)
+    n = 300
+    o = oracle.OracleSystem(sa, wl.ff)
+    for r in (0, 10):
+        say("plain replica", r)
+        e.mc_steps_plain(r, n)
+        orep = oracle.OracleReplica(o, lad[r], wl.translate_step, wl.rotate_step, r)
+        orec = orep.mc(seed=SEED, nsteps=n, mode=1)
+        st = e.state(r)
+        cnt = e.counters()[r]
+        assert cnt[3] == n
+        if cnt[2] == np.sum(orec["accepted"] == 0) and np.array_equal(st["pos"], orep.pos):
+            continue
+        # a divergence must come from a near-threshold decision
+        assert np.min(orec["margin"]) < 1e-4
+    e.close()
+
+
+def test_exchange_matches_oracle(oracle):
+    wl = synth.config(2)
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED)
+    slots = np.arange(len(lad))
+    for rnd in range(12):
+        e.mc_steps(50)
+        en = e.energies()
+        n = e.exchange()
+        want, nw, _ = oracle.exchange_round(lad, slots, en, rnd % 2, rnd, 0, SEED)
+        got = e.ladder(0)
+        say("exchange round", rnd, "swaps", n)
+        assert n == nw
+        assert np.array_equal(got, want)
+        slots = want
+        T = e.temperatures()
+        assert np.allclose(T[got], lad)
+    e.close()
+
+
+def test_graph_run_equals_eager(oracle):
+    """remc_run (hipGraph replay of MC + exchange rounds) gives bit-identical
+    state to the same calls made eagerly, over several replays."""
+    wl = synth.config(2)
+    a = engine_for(wl, seed=SEED, block=256)
+    b = engine_for(wl, seed=SEED, block=256)
+    for k in range(3):
+        a.run(4, 100)
+        for _ in range(4):
+            b.mc_steps(100)
+            b.exchange()
+        a.synchronize()
+        b.synchronize()
+        say("graph replay batch", k)
+        assert np.array_equal(a.energies(), b.energies())
+        assert np.array_equal(a.temperatures(), b.temperatures())
+        assert np.array_equal(a.ladder(0), b.ladder(0))
+        for r in (0, 5, 19):
+            assert np.array_equal(a.state(r)["pos"], b.state(r)["pos"])
+    a.close()
+    b.close()
+
+
+def test_single_body_and_all_crowder_systems(oracle):
+    """Edge systems: one body (dE = 0, every move accepted) and crowders only."""
+    from remcdockingsimulator_amd.engine import Engine
+    from remcdockingsimulator_amd.forcefield import ForceField
+
+    rng = np.random.default_rng(8)
+    ff = ForceField.synthetic(8)
+    one = synth.build_system([synth.chain_body(30, rng)], 100.0, rng)
+    crow = synth.build_system([synth.crowder_body() for _ in range(40)], 60.0, rng)
+    e = Engine(0)
+    e.set_forcefield(ff)
+    e.set_seed(5)
+    s1 = e.add_system(one)
+    e.add_replicas(s1, [300.0, 310.0], 2.0, 0.2)
+    s2 = e.add_system(crow)
+    e.add_replicas(s2, [300.0, 320.0, 340.0], 2.0, 0.2)
+    e.set_recording(200)
+    e.mc_steps(200)
+    e.synchronize()
+    cnt = e.counters()
+    assert np.all(cnt[:2, 2] == 0)  # single body: no rejections
+    assert np.all(e.energies()[:2] == 0.0)
+    o = oracle.OracleSystem(crow, ff)
+    for r in range(3):
+        orep = oracle.OracleReplica(o, [300.0, 320.0, 340.0][r], 2.0, 0.2, 2 + r)
+        orec = orep.mc(seed=5, nsteps=200, mode=0)
+        compare_decisions(e.records(2 + r), orec, [300.0, 320.0, 340.0][r])
+    e.close()
