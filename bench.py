@@ -1,0 +1,255 @@
+#!/usr/bin/env python3
+"""Benchmark of the REMC hot path on one MI355X.
+
+A "step" is one MC step of every replica of the workload (proposal, dE of the
+moved body, Metropolis), with a temperature-exchange round every
+--steps-per-launch steps; the batched path runs them as hipGraph replays of
+(persistent MC kernel + exchange kernel).  Inputs are resident in HBM before
+the timed region.
+
+Default workload: config 4 (10 x 100-residue chains + 300 crowders = 1300
+beads in 310 rigid bodies, 32 replicas 288-480 K), the configuration the
+throughput target is stated on.  Reported beside it: the plain path (the
+reference's per-step protocol with the reference-shaped tiled kernel) on the
+same workload, and the CPU baseline (the FP64 C restatement of the reference
+CPU path, full recomputation per step, replicas block-partitioned over
+pthreads) on a bounded sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+if REPO not in sys.path:
+    sys.path.insert(0, REPO)
+
+import numpy as np  # noqa: E402
+
+from remcdockingsimulator_amd import forcefield as ffm  # noqa: E402
+from remcdockingsimulator_amd import rng as rngm  # noqa: E402
+from remcdockingsimulator_amd import synth  # noqa: E402
+from remcdockingsimulator_amd.engine import Engine  # noqa: E402
+
+METRIC = "MC steps/sec & pair-interactions/sec, batched vs plain, |ΔE|<1e-4 kT vs FP64"
+FLOPS_PER_PAIR = 41          # fixed counting rule per pair term (FMA = 2, sqrt/rcp/exp/rint = 1)
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 vector peak (datasheet)
+SEED = 0x52454D43
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def workload_label(cid, wl):
+    sa = wl.systems[0]
+    labels = {
+        1: "config1: 76+24-residue chains, 150 A box",
+        2: "config2: 76+24-residue chains + 200 crowders, 300 A box",
+        3: "config3: 64 uneven systems (2-12 chains of 30-300 residues, 0-500 crowders)",
+        4: "config4: 10 x 100-residue chains + 300 crowders",
+        5: "config5: 60 x 150-residue chains + 2000 crowders, 600 A box",
+    }
+    return "%s; %d replicas; %s" % (labels[cid], wl.n_replicas,
+                                    "%d beads / %d bodies / box %.1f A" % (sa.n_beads, sa.n_bodies, sa.box)
+                                    if len(wl.systems) == 1 else "%d systems" % len(wl.systems))
+
+
+def read_traffic(kernel, workload):
+    path = os.path.join(REPO, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("hbm_bytes_per_launch")
+
+
+def cpu_baseline(wl, nsteps, threads, gpu_records):
+    """The oracle (FP64 C restatement of the reference CPU path, full energy
+    recomputation per step, Replica.cpp:435) timed on this host, plus the
+    |dE| comparison of the GPU's recorded steps against its FP64 values."""
+    from oracle import pyoracle
+
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    osys = pyoracle.OracleSystem(sa, wl.ff)
+    reps = [pyoracle.OracleReplica(osys, lad[r], wl.translate_step, wl.rotate_step, r) for r in range(len(lad))]
+    secs, rec = pyoracle.mc_threaded(osys, reps, SEED, nsteps, 1, threads, record=True)
+    n_steps = len(reps) * nsteps
+    pairs = n_steps * wl.pairs_full()
+    worst = 0.0
+    agree = 0
+    total = 0
+    for r in range(len(lad)):
+        g = gpu_records[r][:nsteps]
+        o = rec[r]
+        kt = ffm.kT(lad[r])
+        for k in range(nsteps):
+            total += 1
+            if g["body"][k] != o["body"][k] or g["accepted"][k] != o["accepted"][k]:
+                break  # chains diverged at a near-threshold decision: stop comparing
+            worst = max(worst, abs(g["delta"][k] - o["delta"][k]) / kt)
+            agree += 1
+    return {
+        "value": pairs / secs,
+        "unit": "pair-interactions/s",
+        "mc_steps_per_s": n_steps / secs,
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d replicas x %d MC steps of the same workload, full-energy recomputation per step "
+                  "(%d pair terms each), FP64, %d pthreads" % (len(reps), nsteps, wl.pairs_full(), threads),
+        "seconds": secs,
+        "max_abs_dE_err_kT": worst,
+        "decisions_compared": total,
+        "decisions_equal": agree,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000, help="timed MC steps per replica")
+    ap.add_argument("--warmup", type=int, default=1000, help="untimed MC steps per replica (recorded)")
+    ap.add_argument("--config", type=int, default=4)
+    ap.add_argument("--steps-per-launch", type=int, default=1000, help="MC steps between exchanges")
+    ap.add_argument("--plain-steps", type=int, default=300)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--block", type=int, default=0, help="MC workgroup size (0 = autotune)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every replica of the workload runs on a single MI355X; "
+              "--gpus %d is not supported (multi-GPU runs are out of scope)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+
+        n = args.copy_calibration * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n}))
+        return
+
+    cid = args.config
+    wl = synth.config(cid)
+    R = wl.n_replicas
+    spl = args.steps_per_launch
+    if args.steps % spl:
+        raise SystemExit("--steps must be a multiple of --steps-per-launch")
+    rounds = args.steps // spl
+    sizes = [np.diff(sa.body_start) for sa in wl.systems]
+
+    e = Engine(0)
+    synth.load_engine(e, wl, seed=SEED)
+    if args.block:
+        e.set_mc_block(args.block)
+    rec_n = max(args.warmup, 1)
+    e.set_recording(rec_n)
+    log("workload:", workload_label(cid, wl))
+    e.mc_steps(args.warmup)
+    e.synchronize()
+    gpu_records = [e.records(r) for r in range(R)] if len(wl.systems) == 1 else None
+    e.set_recording(0)
+    e.run(1, spl)          # capture + one untimed replay
+    e.synchronize()
+    warm_steps = args.warmup + spl
+    log("mc block:", e.mc_block())
+
+    # ---------------------------------------------------------- timed ----
+    e.timing_enable(True)
+    e.synchronize()
+    t0 = time.perf_counter()
+    e.run(rounds, spl)
+    e.synchronize()
+    t1 = time.perf_counter()
+    kms, nl = e.timing(Engine.MC)
+    wall = t1 - t0
+
+    # exact pair terms evaluated in the timed region (counter-based RNG replayed on the host)
+    pairs = 0
+    rid = 0
+    for s, sa in enumerate(wl.systems):
+        n_rep = len(wl.temperatures[s])
+        pairs += rngm.delta_pairs(SEED, range(rid, rid + n_rep), warm_steps, args.steps, sizes[s])
+        rid += n_rep
+    value = pairs / wall
+    steps_total = R * args.steps
+    achieved = pairs * FLOPS_PER_PAIR / (kms / 1e3) / 1e12
+    traffic = read_traffic("k_mc", "config%d" % cid)
+    roofline = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "kernel": "k_mc (persistent batched MC, dE of the moved body)",
+                "kernel_avg_ms": kms / max(nl, 1), "launches": nl,
+                "flops_per_pair": FLOPS_PER_PAIR, "pairs_per_launch": pairs / max(nl, 1)}
+    energies = e.energies()
+    log("timed: %.3f s, %d launches, kernel %.3f ms/launch" % (wall, nl, kms / max(nl, 1)))
+
+    # ---------------------------------------------------------- plain ----
+    plain = None
+    if not args.no_plain and len(wl.systems) == 1:
+        e.timing_enable(True)
+        e.synchronize()
+        t2 = time.perf_counter()
+        e.mc_steps_plain(0, args.plain_steps)
+        t3 = time.perf_counter()
+        pms, pnl = e.timing(Engine.PLAIN)
+        pw = t3 - t2
+        pf = wl.pairs_full()
+        plain = {"pair_interactions_per_s": args.plain_steps * pf / pw,
+                 "mc_steps_per_s": args.plain_steps / pw,
+                 "kernel_avg_ms": pms / max(pnl, 1),
+                 "kernel_pair_interactions_per_s": pf / (pms / max(pnl, 1) / 1e3),
+                 "protocol": "reference per-step protocol: host proposal, H2D moved body, tiled kernel "
+                             "(lower triangle culled, FP32 partials), D2H partials, host FP64 sum and Metropolis",
+                 "steps": args.plain_steps}
+        log("plain: %.3f s for %d steps" % (pw, args.plain_steps))
+
+    # ------------------------------------------------------------ cpu ----
+    cpu = None
+    if not args.no_cpu and len(wl.systems) == 1 and gpu_records is not None:
+        threads = args.cpu_threads or min(R, os.cpu_count() or 1)
+        cpu = cpu_baseline(wl, min(args.cpu_steps, rec_n), threads, gpu_records)
+        log("cpu: %.2f s" % cpu["seconds"])
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 pair terms, Kahan per thread, f64 totals/Metropolis",
+        "data": "synthetic (seeded self-avoiding chains, uniform residue types, seeded [-0.6,0] kT contact table)",
+        "config": {"workload": workload_label(cid, wl), "replicas": R, "steps_per_exchange": spl,
+                   "mc_block": e.mc_block(), "form": "batched persistent dE kernel, hipGraph rounds"},
+        "mc_steps_per_s": steps_total / wall,
+        "pair_interactions_per_s": value,
+        "roofline": roofline,
+        "plain": plain,
+        "speedup_vs_plain": {"pair_interactions": value / plain["pair_interactions_per_s"],
+                             "mc_steps": (steps_total / wall) / plain["mc_steps_per_s"]} if plain else None,
+        "cpu_baseline": cpu,
+        "energy_mean": float(np.mean(energies)),
+    }
+    e.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

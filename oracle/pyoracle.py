@@ -101,7 +101,7 @@ def lib():
         L.orc_mc_threaded.restype = ctypes.c_double
         L.orc_mc_threaded.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield),
                                       ctypes.POINTER(orc_replica), ctypes.c_int, _u32p, ctypes.c_int,
-                                      ctypes.c_int, ctypes.c_int]
+                                      ctypes.c_int, ctypes.c_int, ctypes.POINTER(orc_step_record)]
         L.orc_propose.restype = None
         L.orc_propose.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_replica), ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_double, ctypes.c_double] + [_fp] * 6 + [_fp, _dp]
@@ -243,14 +243,17 @@ class OracleReplica:
         return self.r.energy
 
 
-def mc_threaded(osys, reps, seed, nsteps, mode, threads):
+def mc_threaded(osys, reps, seed, nsteps, mode, threads, record=False):
+    """Runs nsteps on every replica with block-partitioned threads; returns
+    (wall seconds, records (n_rep, nsteps) or None)."""
     arr = (orc_replica * len(reps))(*[r.r for r in reps])
+    rec = np.zeros((len(reps), nsteps), dtype=RECORD_DTYPE) if record else None
     t = lib().orc_mc_threaded(ctypes.byref(osys.s), ctypes.byref(osys.ff), arr, len(reps), key_words(seed), nsteps,
-                              mode, threads)
+                              mode, threads, rec.ctypes.data_as(ctypes.POINTER(orc_step_record)) if record else None)
     for k, r in enumerate(reps):
         r.r = orc_replica.from_buffer_copy(arr[k])
         r.sync_out()
-    return t
+    return t, rec
 
 
 def exchange_round(ladder, slot_replica, energies, offset, round_, system_id, seed):

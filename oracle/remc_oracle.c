@@ -477,18 +477,21 @@ typedef struct orc_thread_arg {
     orc_replica *reps;
     int first, count, nsteps, mode;
     const uint32_t *key;
+    orc_step_record *rec;
 } orc_thread_arg;
 
 static void *orc_thread_main(void *p)
 {
     orc_thread_arg *a = (orc_thread_arg *)p;
     for (int i = 0; i < a->count; ++i)
-        orc_mc_steps(a->s, a->ff, &a->reps[a->first + i], a->key, a->nsteps, a->mode, NULL);
+        orc_mc_steps(a->s, a->ff, &a->reps[a->first + i], a->key, a->nsteps, a->mode,
+                     a->rec ? a->rec + (size_t)(a->first + i) * a->nsteps : NULL);
     return NULL;
 }
 
 double orc_mc_threaded(const orc_system *s, const orc_forcefield *ff, orc_replica *reps,
-                       int n_rep, const uint32_t key[2], int nsteps, int mode, int threads)
+                       int n_rep, const uint32_t key[2], int nsteps, int mode, int threads,
+                       orc_step_record *rec)
 {
     if (threads < 1) threads = 1;
     if (threads > n_rep) threads = n_rep;
@@ -503,7 +506,7 @@ double orc_mc_threaded(const orc_system *s, const orc_forcefield *ff, orc_replic
         int first = t * per;
         int count = n_rep - first < per ? n_rep - first : per;
         if (count <= 0) break;
-        args[t] = (orc_thread_arg){s, ff, reps, first, count, nsteps, mode, key};
+        args[t] = (orc_thread_arg){s, ff, reps, first, count, nsteps, mode, key, rec};
         pthread_create(&th[t], NULL, orc_thread_main, &args[t]);
         ++launched;
     }

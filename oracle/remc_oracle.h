@@ -174,7 +174,8 @@ int orc_exchange_round(int n_rep, const double *ladder, int *slot_replica,
 /* Threaded CPU baseline: runs nsteps on each replica with block-partitioned
  * pthreads (main.cpp:879-889).  Returns wall seconds. */
 double orc_mc_threaded(const orc_system *s, const orc_forcefield *ff, orc_replica *reps,
-                       int n_rep, const uint32_t key[2], int nsteps, int mode, int threads);
+                       int n_rep, const uint32_t key[2], int nsteps, int mode, int threads,
+                       orc_step_record *rec /* n_rep*nsteps or NULL */);
 
 #ifdef __cplusplus
 }
