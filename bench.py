@@ -121,6 +121,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="MC workgroup size (0 = autotune)")
+    ap.add_argument("--coop", type=int, default=0, help="MC workgroups per replica (0 = autotune)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -155,6 +156,8 @@ def main():
     synth.load_engine(e, wl, seed=SEED)
     if args.block:
         e.set_mc_block(args.block)
+    if args.coop:
+        e.set_mc_coop(args.coop)
     rec_n = max(args.warmup, 1)
     e.set_recording(rec_n)
     log("workload:", workload_label(cid, wl))
@@ -165,7 +168,7 @@ def main():
     e.run(1, spl)          # capture + one untimed replay
     e.synchronize()
     warm_steps = args.warmup + spl
-    log("mc block:", e.mc_block())
+    log("mc block:", e.mc_block(), "workgroups per replica:", e.mc_coop())
 
     # ---------------------------------------------------------- timed ----
     e.timing_enable(True)
@@ -237,7 +240,8 @@ def main():
         "dtype": "f32 pair terms, Kahan per thread, f64 totals/Metropolis",
         "data": "synthetic (seeded self-avoiding chains, uniform residue types, seeded [-0.6,0] kT contact table)",
         "config": {"workload": workload_label(cid, wl), "replicas": R, "steps_per_exchange": spl,
-                   "mc_block": e.mc_block(), "form": "batched persistent dE kernel, hipGraph rounds"},
+                   "mc_block": e.mc_block(), "mc_workgroups_per_replica": e.mc_coop(),
+                   "form": "batched persistent dE kernel, hipGraph rounds"},
         "mc_steps_per_s": steps_total / wall,
         "pair_interactions_per_s": value,
         "roofline": roofline,

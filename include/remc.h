@@ -192,6 +192,13 @@ REMC_API int remc_synchronize(remc_ctx *ctx);
  * 256, 512 or 1024). */
 REMC_API int remc_set_mc_block(remc_ctx *ctx, int block);
 REMC_API int remc_get_mc_block(remc_ctx *ctx, int *block);
+/* Workgroups per replica of the MC kernel (0 = autotune; else 1, 2, 4 or 8).
+ * With more than one, each workgroup keeps a private copy of its replica's
+ * state, evaluates dE over a slice of the partner beads, and the members add
+ * their partials through device memory every step; the grid must fit
+ * co-resident on the device (checked). */
+REMC_API int remc_set_mc_coop(remc_ctx *ctx, int coop);
+REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
 
 #ifdef __cplusplus
 }

@@ -80,6 +80,8 @@ SIGNATURES = {
     "remc_synchronize": (ctypes.c_int, [_vp]),
     "remc_set_mc_block": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_get_mc_block": (ctypes.c_int, [_vp, _ip]),
+    "remc_set_mc_coop": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "remc_get_mc_coop": (ctypes.c_int, [_vp, _ip]),
 }
 
 _lib = None

@@ -111,10 +111,15 @@ struct remc_ctx {
     remc_step_record *d_rec = nullptr;
     unsigned long long *d_rec_start = nullptr;
     int rec_max = 0;
+    unsigned long long *d_slots = nullptr;
+    int *d_err = nullptr;
     DevFF devff{};
 
-    // MC configuration
+    // MC configuration: workgroup size and workgroups per replica (0 = autotune)
     int mc_block = 0;
+    int mc_coop = 0;
+    int synced_copies = 1;   // state copies [0, synced_copies) hold the current conformation
+    int n_cu = 0;
 
     // graph
     hipGraphExec_t graph = nullptr;
@@ -142,6 +147,11 @@ struct remc_ctx {
         p.E = d_E; p.T = d_T; p.step = d_step; p.cnt = d_cnt;
         p.slot_rep = d_slot; p.ladder = d_ladder; p.ex_round = d_exround; p.ex_offset = d_exoff;
         p.ex_swaps = d_exswaps;
+        p.n_rep = (int)reps.size();
+        p.copy_beads = (size_t)total_rep_beads;
+        p.copy_bodies = (size_t)total_rep_bodies;
+        p.slots = d_slots;
+        p.err = d_err;
         p.rec = rec_max > 0 ? d_rec : nullptr;
         p.rec_start = d_rec_start;
         p.rec_max = rec_max;
@@ -325,16 +335,19 @@ int finalize(remc_ctx *c)
     HIP_TRY(dalloc(&c->d_sys, dsys.size()));
     HIP_TRY(dalloc(&c->d_rep, drep.size()));
     HIP_TRY(dalloc(&c->d_etab, etab.size()));
-    HIP_TRY(dalloc(&c->d_px, px.size()));
-    HIP_TRY(dalloc(&c->d_py, py.size()));
-    HIP_TRY(dalloc(&c->d_pz, pz.size()));
-    HIP_TRY(dalloc(&c->d_rx, rx.size()));
-    HIP_TRY(dalloc(&c->d_ry, ry.size()));
-    HIP_TRY(dalloc(&c->d_rz, rz.size()));
-    HIP_TRY(dalloc(&c->d_cx, cx.size()));
-    HIP_TRY(dalloc(&c->d_cy, cy.size()));
-    HIP_TRY(dalloc(&c->d_cz, cz.size()));
-    HIP_TRY(dalloc(&c->d_quat, quat.size()));
+    // REMC_MAX_COOP copies of the dynamic state; copy 0 is the canonical one
+    HIP_TRY(dalloc(&c->d_px, px.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_py, py.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_pz, pz.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_rx, rx.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_ry, ry.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_rz, rz.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_cx, cx.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_cy, cy.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_cz, cz.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_quat, quat.size() * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_slots, (size_t)R * 2 * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_err, (size_t)1));
     HIP_TRY(dalloc(&c->d_E, (size_t)R));
     HIP_TRY(dalloc(&c->d_T, (size_t)R));
     HIP_TRY(dalloc(&c->d_step, (size_t)R));
@@ -380,7 +393,17 @@ int finalize(remc_ctx *c)
     HIP_TRY(upload(c->d_exround, zeros_s, s));
     HIP_TRY(upload(c->d_exswaps, zeros_s, s));
     HIP_TRY(upload(c->d_exoff, zeros_si, s));
+    {
+        std::vector<unsigned long long> empty((size_t)R * 2 * REMC_MAX_COOP, ~0ull);
+        HIP_TRY(upload(c->d_slots, empty, s));
+        int zero = 0;
+        HIP_TRY(hipMemcpyAsync(c->d_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, c->device));
+    c->n_cu = prop.multiProcessorCount;
     HIP_TRY(hipStreamSynchronize(s));
+    c->synced_copies = 1;
     c->finalized = true;
 
     // initial energies of every replica
@@ -406,9 +429,62 @@ int finalize(remc_ctx *c)
         if ((r) < 0 || (r) >= (int)(c)->reps.size()) return fail(REMC_ERR_ARG, "replica %d out of range", (r)); \
     } while (0)
 
+// Copies [synced_copies, coop) of the dynamic state are refreshed from copy 0.
+int sync_copies(remc_ctx *c, int coop)
+{
+    if (c->synced_copies >= coop) return REMC_OK;
+    const size_t nb = (size_t)c->total_rep_beads, nm = (size_t)c->total_rep_bodies;
+    float *fb[6] = {c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz};
+    float *fm[3] = {c->d_cx, c->d_cy, c->d_cz};
+    for (int k = c->synced_copies; k < coop; ++k) {
+        for (float *a : fb) HIP_TRY(hipMemcpyAsync(a + k * nb, a, nb * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        for (float *a : fm) HIP_TRY(hipMemcpyAsync(a + k * nm, a, nm * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        HIP_TRY(hipMemcpyAsync(c->d_quat + 4 * k * nm, c->d_quat, 4 * nm * sizeof(double), hipMemcpyDeviceToDevice,
+                               c->stream));
+    }
+    c->synced_copies = coop;
+    return REMC_OK;
+}
+
+// Reads and clears the device error flag of the cooperative MC kernel.
+int check_err(remc_ctx *c)
+{
+    int err = 0;
+    HIP_TRY(hipMemcpyAsync(&err, c->d_err, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (err) {
+        int zero = 0;
+        HIP_TRY(hipMemcpyAsync(c->d_err, &zero, sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        return fail(REMC_ERR_DEVICE, "cooperative MC workgroups did not meet within the wait limit; replica state is invalid");
+    }
+    return REMC_OK;
+}
+
+// Largest workgroups-per-replica count whose grid is co-resident.
+int max_coop(remc_ctx *c, int block, int *out)
+{
+    int per_cu = 0;
+    HIP_TRY(mc_max_active_per_cu(block, &per_cu));
+    const int R = (int)c->reps.size();
+    int m = REMC_MAX_COOP;
+    while (m > 1 && (long long)m * R > (long long)per_cu * c->n_cu) m >>= 1;
+    *out = m;
+    return REMC_OK;
+}
+
+int mc_launch(remc_ctx *c, const KParams &p, int nsteps, int block, int coop)
+{
+    int rc = sync_copies(c, coop);
+    if (rc != REMC_OK) return rc;
+    HIP_TRY(launch_mc(p, (int)c->reps.size(), nsteps, block, coop, c->stream));
+    c->synced_copies = coop;
+    return REMC_OK;
+}
+
 int autotune_mc(remc_ctx *c)
 {
-    // time each workgroup size on the live state, then restore it exactly
+    // time each (workgroup size, workgroups per replica) on the live state, then restore it exactly
     const int R = (int)c->reps.size();
     struct Buf {
         void *dev;
@@ -423,45 +499,67 @@ int autotune_mc(remc_ctx *c)
         {c->d_cz, sizeof(float) * c->total_rep_bodies, nullptr}, {c->d_quat, sizeof(double) * 4 * c->total_rep_bodies, nullptr},
         {c->d_E, sizeof(double) * R, nullptr}, {c->d_step, sizeof(unsigned long long) * R, nullptr},
         {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}};
+    int rc = sync_copies(c, 1);
+    if (rc != REMC_OK) return rc;
     for (auto &b : bufs) {
         HIP_TRY(hipMalloc(&b.save, b.bytes));
         HIP_TRY(hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     KParams p = c->params();
     p.rec = nullptr;
-    const int cands[3] = {256, 512, 1024};
+    const int blocks[3] = {256, 512, 1024};
     double best = 1e30;
-    int best_block = 256;
+    int best_block = 256, best_coop = 1;
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    int trial_steps = 64;
-    for (int ci = 0; ci < 3; ++ci) {
-        const int blk = cands[ci];
-        HIP_TRY(launch_mc(p, R, 4, blk, c->stream)); // warm
-        HIP_TRY(hipEventRecord(e0, c->stream));
-        HIP_TRY(launch_mc(p, R, trial_steps, blk, c->stream));
-        HIP_TRY(hipEventRecord(e1, c->stream));
-        HIP_TRY(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_block = blk;
+    const int trial_steps = 64;
+    for (int bi = 0; bi < 3; ++bi) {
+        const int blk = blocks[bi];
+        if (c->mc_block && blk != c->mc_block) continue;
+        int cmax = 1;
+        rc = max_coop(c, blk, &cmax);
+        if (rc != REMC_OK) return rc;
+        for (int coop = 1; coop <= cmax; coop *= 2) {
+            if (c->mc_coop && coop != c->mc_coop) continue;
+            c->synced_copies = 1;
+            rc = mc_launch(c, p, 4, blk, coop); // warm
+            if (rc != REMC_OK) return rc;
+            HIP_TRY(hipEventRecord(e0, c->stream));
+            rc = mc_launch(c, p, trial_steps, blk, coop);
+            if (rc != REMC_OK) return rc;
+            HIP_TRY(hipEventRecord(e1, c->stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            rc = check_err(c);
+            if (rc != REMC_OK) return rc;
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_block = blk;
+                best_coop = coop;
+            }
+            for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
         }
-        for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     for (auto &b : bufs) hipFree(b.save);
+    c->synced_copies = 1;
     c->mc_block = best_block;
+    c->mc_coop = best_coop;
     return REMC_OK;
 }
 
-int ensure_mc_block(remc_ctx *c)
+int ensure_mc_config(remc_ctx *c)
 {
-    if (c->mc_block == 0) return autotune_mc(c);
+    if (c->mc_block == 0 || c->mc_coop == 0) return autotune_mc(c);
+    int cmax = 1;
+    int rc = max_coop(c, c->mc_block, &cmax);
+    if (rc != REMC_OK) return rc;
+    if (c->mc_coop > cmax)
+        return fail(REMC_ERR_LIMIT, "%d workgroups per replica do not fit co-resident (at most %d)", c->mc_coop, cmax);
     return REMC_OK;
 }
 
@@ -502,7 +600,7 @@ int remc_ctx_destroy(remc_ctx *c)
                     c->d_sys, c->d_rep, c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz,
                     c->d_cx, c->d_cy, c->d_cz, c->d_quat, c->d_E, c->d_T, c->d_step, c->d_cnt,
                     c->d_slot, c->d_exoff, c->d_ladder, c->d_exround, c->d_exswaps, c->d_full_partials,
-                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start};
+                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->free_events) hipEventDestroy(e);
@@ -715,13 +813,15 @@ int remc_mc_steps(remc_ctx *c, int nsteps)
     FINALIZE(c);
     if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
     if (nsteps == 0) return REMC_OK;
-    int rc = ensure_mc_block(c);
+    int rc = ensure_mc_config(c);
+    if (rc != REMC_OK) return rc;
+    rc = sync_copies(c, c->mc_coop);
     if (rc != REMC_OK) return rc;
     {
         TimedLaunch t(c, 0);
-        HIP_TRY(launch_mc(c->params(), (int)c->reps.size(), nsteps, c->mc_block, c->stream));
+        rc = mc_launch(c, c->params(), nsteps, c->mc_block, c->mc_coop);
     }
-    return REMC_OK;
+    return rc;
 }
 
 int remc_mc_steps_plain(remc_ctx *c, int replica, int nsteps, int block)
@@ -822,6 +922,7 @@ int remc_mc_steps_plain(remc_ctx *c, int replica, int nsteps, int block)
     HIP_TRY(hipMemcpyAsync(c->d_step + replica, &step, 8, hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(c->d_cnt + 3 * replica, cnt, 24, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
+    c->synced_copies = 1;
     return REMC_OK;
 }
 
@@ -853,14 +954,16 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
     FINALIZE(c);
     if (rounds < 0 || steps_per_round < 0) return fail(REMC_ERR_ARG, "negative rounds or steps");
     if (rounds == 0) return REMC_OK;
-    int rc = ensure_mc_block(c);
+    int rc = ensure_mc_config(c);
+    if (rc != REMC_OK) return rc;
+    rc = sync_copies(c, c->mc_coop);
     if (rc != REMC_OK) return rc;
     if (!c->graph || c->graph_steps != steps_per_round) {
         destroy_graph(c);
         KParams p = c->params();
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e1 = launch_mc(p, (int)c->reps.size(), steps_per_round, c->mc_block, c->stream);
+        hipError_t e1 = launch_mc(p, (int)c->reps.size(), steps_per_round, c->mc_block, c->mc_coop, c->stream);
         hipError_t e2 = launch_exchange(p, (int)c->systems.size(), c->stream);
         hipError_t e3 = hipStreamEndCapture(c->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -921,7 +1024,7 @@ int remc_get_energies(remc_ctx *c, double *out)
     if (!out) return fail(REMC_ERR_ARG, "null output");
     HIP_TRY(hipMemcpyAsync(out, c->d_E, sizeof(double) * c->reps.size(), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    return REMC_OK;
+    return check_err(c);
 }
 
 int remc_get_temperatures(remc_ctx *c, double *out)
@@ -1022,6 +1125,7 @@ int remc_set_state(remc_ctx *c, int replica, const float *x, const float *y, con
         if (src_m[i]) HIP_TRY(hipMemcpyAsync(dst_m[i] + mo, src_m[i], 4 * nm, hipMemcpyHostToDevice, st));
     if (quat) HIP_TRY(hipMemcpyAsync(c->d_quat + 4 * (size_t)mo, quat, 32 * nm, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
+    c->synced_copies = 1;
     return REMC_OK;
 }
 
@@ -1060,6 +1164,7 @@ int remc_synchronize(remc_ctx *c)
 {
     CHECK_CTX(c);
     HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->finalized) return check_err(c);
     return REMC_OK;
 }
 
@@ -1070,6 +1175,24 @@ int remc_set_mc_block(remc_ctx *c, int block)
         return fail(REMC_ERR_ARG, "MC block must be 0, 256, 512 or 1024");
     c->mc_block = block;
     destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_set_mc_coop(remc_ctx *c, int coop)
+{
+    CHECK_CTX(c);
+    if (coop != 0 && coop != 1 && coop != 2 && coop != 4 && coop != 8)
+        return fail(REMC_ERR_ARG, "workgroups per replica must be 0, 1, 2, 4 or 8");
+    c->mc_coop = coop;
+    destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_get_mc_coop(remc_ctx *c, int *coop)
+{
+    CHECK_CTX(c);
+    if (!coop) return fail(REMC_ERR_ARG, "null output");
+    *coop = c->mc_coop;
     return REMC_OK;
 }
 

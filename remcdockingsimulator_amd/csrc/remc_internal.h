@@ -73,6 +73,13 @@ struct KParams {
     unsigned long long *ex_round;  // per system
     int *ex_offset;         // per system
     unsigned long long *ex_swaps;  // per system cumulative
+    // cooperative MC: private state copies and partial-dE slots
+    int n_rep;
+    int pad_coop;
+    size_t copy_beads;      // stride between state copies (beads)
+    size_t copy_bodies;     // stride between state copies (bodies)
+    unsigned long long *slots; // n_rep * 2 * REMC_MAX_COOP
+    int *err;               // set when a member waited too long
     // recording
     remc_step_record *rec;
     unsigned long long *rec_start;
@@ -81,7 +88,8 @@ struct KParams {
 };
 
 // launchers (remc_kernels.hip)
-hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s);
+hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
+hipError_t mc_max_active_per_cu(int block, int *out);
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
                               int noncrowder_only, int write_E, hipStream_t s);
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
@@ -90,5 +98,6 @@ hipError_t launch_adjoint(const KParams &p, int replica, int n_beads, int n_bodi
                           double *param_grad, hipStream_t s);
 
 constexpr int FULL_TILE = 256;
+constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
 
 } // namespace remc

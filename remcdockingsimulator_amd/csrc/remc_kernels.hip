@@ -77,9 +77,55 @@ __device__ __forceinline__ double wave_sum(double v)
 
 // --------------------------------------------------------------------------
 // batched persistent MC
+//
+// Grid = coop * n_rep workgroups; workgroup b serves replica r = b % n_rep as
+// member c = b / n_rep of its cooperative group (so with n_rep a multiple of 8
+// the members of one replica share an XCD and its L2).  Member c keeps a
+// private copy c of the replica's dynamic state and evaluates dE over the
+// partner-bead slice [c*N/coop, (c+1)*N/coop).  Members publish their partial
+// dE per step in a 64-bit slot (value with its low 20 bits replaced by the step
+// tag, written with one atomic store) and every member adds the coop partials
+// in member order, so all members take the identical decision and update
+// their copies identically.  With coop = 1 no slot is used.  A member that
+// waits longer than a bounded number of polls sets the error flag; every
+// member that sees the flag leaves the step loop, so the grid always drains.
 // --------------------------------------------------------------------------
+constexpr unsigned long long SLOT_TAG_MASK = 0xFFFFFull;   // 0xFFFFF = empty slot
+constexpr int SLOT_SPIN_LIMIT = 1 << 22;
+
+__device__ __forceinline__ unsigned long long slot_pack(double v, unsigned long long tag)
+{
+    return ((unsigned long long)__double_as_longlong(v) & ~SLOT_TAG_MASK) | tag;
+}
+
+__device__ __forceinline__ double slot_value(unsigned long long w)
+{
+    return __longlong_as_double((long long)(w & ~SLOT_TAG_MASK));
+}
+
+// Number of k-groups (power of two <= waves) that balances the
+// (moved bead x partner bead) work of one step across the workgroup.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
+__device__ __forceinline__ int pick_kgroups(int nm, int nj)
+{
+    constexpr int NW = BLOCK / 64;
+    int best = 1;
+    float beste = -1.0f;
+    for (int tk = 1; tk <= NW && tk <= nm; tk *= 2) {
+        const int G = BLOCK / tk;
+        const int jper = (nj + G - 1) / G;
+        const int kper = (nm + tk - 1) / tk;
+        const float e = ((float)nj / (float)(G * max(jper, 1))) * ((float)nm / (float)(tk * kper));
+        if (e > beste + 1e-4f) {
+            beste = e;
+            best = tk;
+        }
+    }
+    return best;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
@@ -87,15 +133,19 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
     __shared__ float4 s_new[REMC_MAX_BODY_BEADS]; // x,y,z,sig proposed positions
     __shared__ int s_tc[REMC_MAX_BODY_BEADS];
     __shared__ double s_red[NW];
+    __shared__ double s_dE;
+    __shared__ int s_abort;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int r = blockIdx.x;
+    const int r = blockIdx.x % p.n_rep;
+    const int c = blockIdx.x / p.n_rep;
     const DevRep R = p.rep[r];
     const DevSys S = p.sys[R.sys];
     const DevFF ff = p.ff;
 
     for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
+    if (tid == 0) s_abort = 0;
 
     double E = p.E[r];
     const double T = p.T[r];
@@ -103,10 +153,18 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
     unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
     const unsigned long long rec_start = p.rec ? p.rec_start[r] : 0ull;
 
-    const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    // private copy c of this replica's dynamic state
+    const size_t cb = (size_t)c * p.copy_beads + R.bead_off;
+    const size_t cm = (size_t)c * p.copy_bodies + R.body_off;
+    float *gx = p.px + cb, *gy = p.py + cb, *gz = p.pz + cb;
+    float *grx = p.rx + cb, *gry = p.ry + cb, *grz = p.rz + cb;
+    float *gcx = p.cx + cm, *gcy = p.cy + cm, *gcz = p.cz + cm;
+    double *gq = p.quat + 4 * cm;
     const int *tcs = p.tc + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
+    const int js = (int)(((long long)c * N) / coop), je = (int)(((long long)(c + 1) * N) / coop);
+    unsigned long long *slots = p.slots + (size_t)r * 2 * REMC_MAX_COOP;
     __syncthreads();
 
     for (int it = 0; it < nsteps; ++it, ++step) {
@@ -114,9 +172,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         remc_draws d = remc_step_draws(step, R.rng_id, p.key0, p.key1, S.n_bodies);
         const int gb = S.body_off + d.body;
         const int m0 = p.bstart[gb], m1 = p.bend[gb], nm = m1 - m0;
-        const bool cm = p.bcrow[gb] != 0;
-        const int rb = R.body_off + d.body;
-        const float cx = p.cx[rb], cy = p.cy[rb], cz = p.cz[rb];
+        const bool crow_m = p.bcrow[gb] != 0;
+        const float cx = gcx[d.body], cy = gcy[d.body], cz = gcz[d.body];
         double coef[9];
         double qmv[4];
         float ncx = cx, ncy = cy, ncz = cz;
@@ -128,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         }
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
-            float rx = p.rx[R.bead_off + b], ry = p.ry[R.bead_off + b], rz = p.rz[R.bead_off + b];
+            float rx = grx[b], ry = gry[b], rz = grz[b];
             float nrx = rx, nry = ry, nrz = rz;
             if (d.rotate) remc_qrotate(coef, rx, ry, rz, &nrx, &nry, &nrz);
             s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b]);
@@ -137,15 +194,21 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         }
         __syncthreads();
 
-        // dE = sum_{i in body} sum_{j not in body} e(new_i, j) - e(old_i, j)
+        // this member's share of dE = sum_{i in body} sum_{j in slice, not in body} e(new_i,j) - e(old_i,j),
+        // split into tk groups of moved beads x G lanes over partner beads
+        const int ov = max(0, min(je, m1) - max(js, m0));
+        const int tk = pick_kgroups<BLOCK>(nm, (je - js) - ov);
+        const int G = BLOCK / tk;
+        const int g = tid / G, l = tid - g * G;
+        const int k0 = (g * nm) / tk, k1 = ((g + 1) * nm) / tk;
         float acc = 0.0f, comp = 0.0f;
-        for (int j = tid; j < N; j += BLOCK) {
+        for (int j = js + l; j < je; j += G) {
             if (j >= m0 && j < m1) continue;
             const float xj = gx[j], yj = gy[j], zj = gz[j];
             const int tcj = tcs[j];
             const bool cj = (tcj >> 8) != 0;
-            if (cm || cj) {
-                for (int k = 0; k < nm; ++k) {
+            if (crow_m || cj) {
+                for (int k = k0; k < k1; ++k) {
                     const float4 o = s_old[k], n = s_new[k];
                     float dx = min_image(xj - o.x, S.box, S.inv_box);
                     float dy = min_image(yj - o.y, S.box, S.inv_box);
@@ -160,7 +223,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
             } else {
                 const float qj = qs[j] * ff.c_dh, sj = sgs[j];
                 const float *erow = s_etab + (tcj & 0xff);
-                for (int k = 0; k < nm; ++k) {
+                for (int k = k0; k < k1; ++k) {
                     const float4 o = s_old[k], n = s_new[k];
                     const int ti = s_tc[k] & 0xff;
                     const float eij = erow[ti * REMC_NTYPES];
@@ -182,40 +245,78 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         double v = wave_sum((double)acc - (double)comp);
         if (lane == 0) s_red[wave] = v;
         __syncthreads();
-        double dE = 0.0;
+        if (wave == 0) {
+            double part = 0.0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) dE += s_red[w];
+            for (int w = 0; w < NW; ++w) part += s_red[w];
+            if (coop == 1) {
+                if (lane == 0) s_dE = part;
+            } else {
+                const unsigned long long tag = step & 0x7FFFFull;
+                unsigned long long *slot = slots + (step & 1ull) * REMC_MAX_COOP;
+                if (lane == 0) __hip_atomic_store(slot + c, slot_pack(part, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                unsigned long long w = 0ull;
+                bool ok = lane >= coop;
+                int spins = 0;
+                bool failed = false;
+                while (!__all(ok)) {
+                    if (!ok) {
+                        w = __hip_atomic_load(slot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = (w & SLOT_TAG_MASK) == tag;
+                    }
+                    if (++spins >= SLOT_SPIN_LIMIT) { failed = true; break; }
+                    if ((spins & 255) == 0 && __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                        failed = true;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                if (failed) {
+                    if (lane == 0) {
+                        __hip_atomic_fetch_or(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        s_abort = 1;
+                    }
+                } else {
+                    // members' partials summed in member order: identical in every member
+                    double tot = 0.0;
+                    for (int m = 0; m < coop; ++m) tot += slot_value(__shfl(w, m, 64));
+                    if (lane == 0) s_dE = tot;
+                }
+            }
+        }
+        __syncthreads();
+        if (s_abort) break;
+        const double dE = s_dE;
 
         const int decision = remc_metropolis(dE, T, d.u);
         if (decision) {
             for (int k = tid; k < nm; k += BLOCK) {
                 const int b = m0 + k;
                 const float4 n = s_new[k];
-                p.px[R.bead_off + b] = n.x;
-                p.py[R.bead_off + b] = n.y;
-                p.pz[R.bead_off + b] = n.z;
+                gx[b] = n.x;
+                gy[b] = n.y;
+                gz[b] = n.z;
                 if (d.rotate) {
-                    float rx = p.rx[R.bead_off + b], ry = p.ry[R.bead_off + b], rz = p.rz[R.bead_off + b];
                     float nrx, nry, nrz;
-                    remc_qrotate(coef, rx, ry, rz, &nrx, &nry, &nrz);
-                    p.rx[R.bead_off + b] = nrx;
-                    p.ry[R.bead_off + b] = nry;
-                    p.rz[R.bead_off + b] = nrz;
+                    remc_qrotate(coef, grx[b], gry[b], grz[b], &nrx, &nry, &nrz);
+                    grx[b] = nrx;
+                    gry[b] = nry;
+                    grz[b] = nrz;
                 }
             }
             if (tid == 0) {
                 if (d.rotate) {
-                    double qo[4] = {p.quat[4 * rb + 0], p.quat[4 * rb + 1], p.quat[4 * rb + 2], p.quat[4 * rb + 3]};
+                    double qo[4] = {gq[4 * d.body + 0], gq[4 * d.body + 1], gq[4 * d.body + 2], gq[4 * d.body + 3]};
                     double qn[4];
                     remc_qmul(qmv, qo, qn);
-                    p.quat[4 * rb + 0] = qn[0];
-                    p.quat[4 * rb + 1] = qn[1];
-                    p.quat[4 * rb + 2] = qn[2];
-                    p.quat[4 * rb + 3] = qn[3];
+                    gq[4 * d.body + 0] = qn[0];
+                    gq[4 * d.body + 1] = qn[1];
+                    gq[4 * d.body + 2] = qn[2];
+                    gq[4 * d.body + 3] = qn[3];
                 } else {
-                    p.cx[rb] = ncx;
-                    p.cy[rb] = ncy;
-                    p.cz[rb] = ncz;
+                    gcx[d.body] = ncx;
+                    gcy[d.body] = ncy;
+                    gcz[d.body] = ncz;
                 }
             }
             E += dE;
@@ -223,7 +324,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         } else {
             ++n_rej;
         }
-        if (p.rec && tid == 0) {
+        if (c == 0 && p.rec && tid == 0) {
             unsigned long long idx = step - rec_start;
             if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
                 remc_step_record rr;
@@ -237,7 +338,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
         }
         __syncthreads();
     }
-    if (tid == 0) {
+    if (c == 0 && tid == 0 && !s_abort) {
         p.E[r] = E;
         p.step[r] = step;
         p.cnt[3 * r + 0] += n_acc;
@@ -246,16 +347,38 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps)
     }
 }
 
-hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s)
+__global__ void k_reset_slots(unsigned long long *slots, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slots[i] = ~0ull;
+}
+
+hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s)
 {
     if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
+    if (coop < 1 || coop > REMC_MAX_COOP) return hipErrorInvalidValue;
+    if (coop > 1) {
+        const int n = n_rep * 2 * REMC_MAX_COOP;
+        hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
+    }
+    const dim3 grid(n_rep * coop);
     switch (block) {
-    case 256: hipLaunchKernelGGL(k_mc<256>, dim3(n_rep), dim3(256), 0, s, p, nsteps); break;
-    case 512: hipLaunchKernelGGL(k_mc<512>, dim3(n_rep), dim3(512), 0, s, p, nsteps); break;
-    case 1024: hipLaunchKernelGGL(k_mc<1024>, dim3(n_rep), dim3(1024), 0, s, p, nsteps); break;
+    case 256: hipLaunchKernelGGL(k_mc<256>, grid, dim3(256), 0, s, p, nsteps, coop); break;
+    case 512: hipLaunchKernelGGL(k_mc<512>, grid, dim3(512), 0, s, p, nsteps, coop); break;
+    case 1024: hipLaunchKernelGGL(k_mc<1024>, grid, dim3(1024), 0, s, p, nsteps, coop); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t mc_max_active_per_cu(int block, int *out)
+{
+    switch (block) {
+    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<256>, 256, 0);
+    case 512: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<512>, 512, 0);
+    case 1024: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<1024>, 1024, 0);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 // --------------------------------------------------------------------------

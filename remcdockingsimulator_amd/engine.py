@@ -153,6 +153,14 @@ class Engine:
         self._chk(self.lib.remc_get_mc_block(self.ctx, ctypes.byref(b)), "remc_get_mc_block")
         return b.value
 
+    def set_mc_coop(self, coop):
+        self._chk(self.lib.remc_set_mc_coop(self.ctx, int(coop)), "remc_set_mc_coop")
+
+    def mc_coop(self):
+        b = ctypes.c_int()
+        self._chk(self.lib.remc_get_mc_coop(self.ctx, ctypes.byref(b)), "remc_get_mc_coop")
+        return b.value
+
     def adjoint(self, replica):
         m = self.systems[self.replica_sys[replica]].n_bodies
         ft = np.zeros((m, 6), dtype=np.float64)

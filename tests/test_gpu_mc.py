@@ -19,11 +19,13 @@ pytestmark = pytest.mark.gpu
 SEED = 0xC0FFEE
 
 
-def run_parity(oracle, cid, nsteps, block=None, replicas=None):
+def run_parity(oracle, cid, nsteps, block=None, replicas=None, coop=None):
     wl = synth.config(cid)
     sa = wl.systems[0]
     lad = wl.temperatures[0]
     e = engine_for(wl, seed=SEED, block=block)
+    if coop:
+        e.set_mc_coop(coop)
     e.set_recording(nsteps)
     e.mc_steps(nsteps)
     e.synchronize()
@@ -68,7 +70,40 @@ def test_mc_parity_config4(oracle):
 
 @pytest.mark.parametrize("block", [256, 512, 1024])
 def test_mc_block_sizes_agree(oracle, block):
-    assert run_parity(oracle, 2, 400, block=block, replicas=[0, 7, 19]) <= 1
+    assert run_parity(oracle, 2, 400, block=block, replicas=[0, 7, 19], coop=1) <= 1
+
+
+@pytest.mark.parametrize("coop", [2, 4, 8])
+def test_mc_cooperative_workgroups(oracle, coop):
+    """Several workgroups per replica, each on a slice of the partner beads,
+    exchanging partial dE every step: same decisions and states."""
+    assert run_parity(oracle, 4, 300, block=512, replicas=range(0, 32, 8), coop=coop) <= 1
+    assert run_parity(oracle, 1, 500, block=256, coop=coop) <= 1
+
+
+def test_cooperative_state_resync_after_host_changes(oracle):
+    """Plain (host-protocol) steps and set_state change copy 0 only; the next
+    cooperative launch must refresh the other copies first."""
+    wl = synth.config(2)
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED, block=256)
+    e.set_mc_coop(4)
+    e.mc_steps(100)
+    e.mc_steps_plain(3, 50)
+    e.set_recording(200)
+    e.mc_steps(200)
+    e.synchronize()
+    o = oracle.OracleSystem(sa, wl.ff)
+    orep = oracle.OracleReplica(o, lad[3], wl.translate_step, wl.rotate_step, 3)
+    orec = orep.mc(seed=SEED, nsteps=150, mode=0)
+    orep2 = oracle.OracleReplica(o, lad[3], wl.translate_step, wl.rotate_step, 3, pos=orep.pos, rel=orep.rel,
+                                 center=orep.center, quat=orep.quat, energy=orep.energy, step=150)
+    orec2 = orep2.mc(seed=SEED, nsteps=200, mode=0)
+    n_eq, div = compare_decisions(e.records(3), orec2, lad[3])
+    if not div:
+        assert np.array_equal(e.state(3)["pos"], orep2.pos)
+    e.close()
 
 
 def test_plain_protocol_matches_oracle(oracle):
@@ -121,6 +156,8 @@ def test_graph_run_equals_eager(oracle):
     wl = synth.config(2)
     a = engine_for(wl, seed=SEED, block=256)
     b = engine_for(wl, seed=SEED, block=256)
+    a.set_mc_coop(4)
+    b.set_mc_coop(4)
     for k in range(3):
         a.run(4, 100)
         for _ in range(4):
