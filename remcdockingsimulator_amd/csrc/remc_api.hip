@@ -118,6 +118,7 @@ struct remc_ctx {
     // MC configuration: workgroup size and workgroups per replica (0 = autotune)
     int mc_block = 0;
     int mc_coop = 0;
+    int coop_min_pairs = 8192; // moves below this many pair terms are evaluated whole by every member
     int synced_copies = 1;   // state copies [0, synced_copies) hold the current conformation
     int n_cu = 0;
 
@@ -148,6 +149,7 @@ struct remc_ctx {
         p.slot_rep = d_slot; p.ladder = d_ladder; p.ex_round = d_exround; p.ex_offset = d_exoff;
         p.ex_swaps = d_exswaps;
         p.n_rep = (int)reps.size();
+        p.coop_min_pairs = coop_min_pairs;
         p.copy_beads = (size_t)total_rep_beads;
         p.copy_bodies = (size_t)total_rep_bodies;
         p.slots = d_slots;

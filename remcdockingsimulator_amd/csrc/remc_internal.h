@@ -75,7 +75,7 @@ struct KParams {
     unsigned long long *ex_swaps;  // per system cumulative
     // cooperative MC: private state copies and partial-dE slots
     int n_rep;
-    int pad_coop;
+    int coop_min_pairs;     // moves with fewer pair terms are not split across members
     size_t copy_beads;      // stride between state copies (beads)
     size_t copy_bodies;     // stride between state copies (bodies)
     unsigned long long *slots; // n_rep * 2 * REMC_MAX_COOP

@@ -163,8 +163,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     const int *tcs = p.tc + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
-    const int js = (int)(((long long)c * N) / coop), je = (int)(((long long)(c + 1) * N) / coop);
+    const int cjs = (int)(((long long)c * N) / coop), cje = (int)(((long long)(c + 1) * N) / coop);
     unsigned long long *slots = p.slots + (size_t)r * 2 * REMC_MAX_COOP;
+    unsigned int n_split = 0; // split steps so far in this launch: selects the slot parity
     __syncthreads();
 
     for (int it = 0; it < nsteps; ++it, ++step) {
@@ -194,6 +195,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         }
         __syncthreads();
 
+        // Moves with few pair terms are evaluated whole by every member (no
+        // exchange); larger ones are split over the members' partner slices.
+        const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
+        const int js = split ? cjs : 0, je = split ? cje : N;
         // this member's share of dE = sum_{i in body} sum_{j in slice, not in body} e(new_i,j) - e(old_i,j),
         // split into tk groups of moved beads x G lanes over partner beads
         const int ov = max(0, min(je, m1) - max(js, m0));
@@ -245,15 +250,21 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         double v = wave_sum((double)acc - (double)comp);
         if (lane == 0) s_red[wave] = v;
         __syncthreads();
+        double dE = 0.0;
+        if (!split) {
+#pragma unroll
+            for (int w = 0; w < NW; ++w) dE += s_red[w];
+        } else {
         if (wave == 0) {
             double part = 0.0;
 #pragma unroll
             for (int w = 0; w < NW; ++w) part += s_red[w];
-            if (coop == 1) {
-                if (lane == 0) s_dE = part;
-            } else {
+            {
+                // slots alternate with the split-step count: a member reuses a
+                // slot only after every member has read it (they all wrote
+                // their partial of the split step in between)
                 const unsigned long long tag = step & 0x7FFFFull;
-                unsigned long long *slot = slots + (step & 1ull) * REMC_MAX_COOP;
+                unsigned long long *slot = slots + (n_split & 1u) * REMC_MAX_COOP;
                 if (lane == 0) __hip_atomic_store(slot + c, slot_pack(part, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 unsigned long long w = 0ull;
                 bool ok = lane >= coop;
@@ -286,7 +297,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         }
         __syncthreads();
         if (s_abort) break;
-        const double dE = s_dE;
+        dE = s_dE;
+        ++n_split;
+        }
 
         const int decision = remc_metropolis(dE, T, d.u);
         if (decision) {
