@@ -164,6 +164,17 @@ def main():
     e.mc_steps(args.warmup)
     e.synchronize()
     gpu_records = [e.records(r) for r in range(R)] if len(wl.systems) == 1 else None
+    step_cost = None
+    if gpu_records is not None:
+        sizes0 = sizes[0]
+        allrec = np.concatenate(gpu_records)
+        nm = sizes0[allrec["body"]]
+        small = nm * (sizes0.sum() - nm) < 8192
+        us = allrec["ticks"] / 100.0  # 100 MHz device clock
+        step_cost = {"small_move_us": float(np.mean(us[small])) if small.any() else None,
+                     "large_move_us": float(np.mean(us[~small])) if (~small).any() else None,
+                     "large_move_fraction": float(np.mean(~small))}
+        log("per-step cost (warmup, recorded):", step_cost)
     e.set_recording(0)
     e.run(1, spl)          # capture + one untimed replay
     e.synchronize()
@@ -244,6 +255,7 @@ def main():
                    "form": "batched persistent dE kernel, hipGraph rounds"},
         "mc_steps_per_s": steps_total / wall,
         "pair_interactions_per_s": value,
+        "step_cost": step_cost,
         "roofline": roofline,
         "plain": plain,
         "speedup_vs_plain": {"pair_interactions": value / plain["pair_interactions_per_s"],

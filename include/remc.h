@@ -89,7 +89,8 @@ typedef struct remc_counters {
 } remc_counters;
 
 typedef struct remc_step_record {
-    int32_t body, move, accepted, pad; /* move 1 = rotate; accepted 0/1/2 */
+    int32_t body, move, accepted; /* move 1 = rotate; accepted 0/1/2 */
+    int32_t ticks;                /* device wall-clock ticks (100 MHz) the step took */
     double delta;                      /* dE, kcal/mol */
 } remc_step_record;
 

@@ -41,11 +41,11 @@ class remc_counters(ctypes.Structure):
 
 class remc_step_record(ctypes.Structure):
     _fields_ = [("body", ctypes.c_int32), ("move", ctypes.c_int32), ("accepted", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("delta", ctypes.c_double)]
+                ("ticks", ctypes.c_int32), ("delta", ctypes.c_double)]
 
 
 RECORD_DTYPE = np.dtype([("body", np.int32), ("move", np.int32), ("accepted", np.int32),
-                         ("pad", np.int32), ("delta", np.float64)])
+                         ("ticks", np.int32), ("delta", np.float64)])
 
 # every entry point declared in include/remc.h: name -> (restype, argtypes)
 _vp = ctypes.c_void_p

@@ -103,26 +103,42 @@ __device__ __forceinline__ double slot_value(unsigned long long w)
     return __longlong_as_double((long long)(w & ~SLOT_TAG_MASK));
 }
 
-// Number of k-groups (power of two <= waves) that balances the
-// (moved bead x partner bead) work of one step across the workgroup.
+// log2 of the number of k-groups (power of two <= waves) that balances the
+// (moved bead x partner bead) work of one step across the workgroup.  All
+// divisions are by powers of two (shifts).
 template <int BLOCK>
-__device__ __forceinline__ int pick_kgroups(int nm, int nj)
+__device__ __forceinline__ int pick_kgroups_log2(int nm, int nj)
 {
     constexpr int NW = BLOCK / 64;
-    int best = 1;
-    float beste = -1.0f;
-    for (int tk = 1; tk <= NW && tk <= nm; tk *= 2) {
-        const int G = BLOCK / tk;
-        const int jper = (nj + G - 1) / G;
-        const int kper = (nm + tk - 1) / tk;
-        const float e = ((float)nj / (float)(G * max(jper, 1))) * ((float)nm / (float)(tk * kper));
-        if (e > beste + 1e-4f) {
-            beste = e;
-            best = tk;
+    constexpr int LB = BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10;
+    // the critical path of a step is the busiest thread: ceil(nj/G) partner
+    // beads times ceil(nm/tk) moved beads; take the smallest product
+    int best = 0, best_work = 0x7fffffff;
+    for (int lt = 0; (1 << lt) <= NW && (1 << lt) <= nm; ++lt) {
+        const int lg = LB - lt; // log2 of the group width
+        const int jper = (nj + (1 << lg) - 1) >> lg;
+        const int kper = (nm + (1 << lt) - 1) >> lt;
+        const int work = jper * kper;
+        if (work < best_work) {
+            best_work = work;
+            best = lt;
         }
     }
     return best;
 }
+
+// Draws of one MC step and the move parameters that depend on them only
+// (formed ahead of the step, one step per lane).
+struct StepDraw {
+    double u;        // Metropolis uniform
+    double coef[9];  // rotation coefficients (remc_qcoeffs) of a rotate move
+    double q[4];     // rotation quaternion of a rotate move
+    float t[3];      // translation vector of a translate move
+    int body;
+    int rotate;
+    int pad;
+};
+constexpr int DRAW_BATCH = 64;
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
@@ -135,6 +151,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
+    __shared__ StepDraw s_draw[DRAW_BATCH];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -169,26 +186,42 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __syncthreads();
 
     for (int it = 0; it < nsteps; ++it, ++step) {
-        // draws and move parameters: every lane computes the same values
-        remc_draws d = remc_step_draws(step, R.rng_id, p.key0, p.key1, S.n_bodies);
-        const int gb = S.body_off + d.body;
+        const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
+        const int slot_d = it & (DRAW_BATCH - 1);
+        if (slot_d == 0) {
+            // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
+            if (wave == 0 && it + lane < nsteps) {
+                const remc_draws dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+                StepDraw &sd = s_draw[lane];
+                sd.u = dd.u;
+                sd.body = dd.body;
+                sd.rotate = dd.rotate;
+                if (dd.rotate) {
+                    double qq[4], cf[9];
+                    remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
+                    remc_qcoeffs(qq, cf);
+                    for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
+                    for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
+                } else {
+                    remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
+                }
+            }
+            __syncthreads();
+        }
+        const StepDraw &D = s_draw[slot_d];
+        const int body = D.body;
+        const bool rotate = D.rotate != 0;
+        const int gb = S.body_off + body;
         const int m0 = p.bstart[gb], m1 = p.bend[gb], nm = m1 - m0;
         const bool crow_m = p.bcrow[gb] != 0;
-        const float cx = gcx[d.body], cy = gcy[d.body], cz = gcz[d.body];
-        double coef[9];
-        double qmv[4];
+        const float cx = gcx[body], cy = gcy[body], cz = gcz[body];
         float ncx = cx, ncy = cy, ncz = cz;
-        if (d.rotate) {
-            remc_move_quat(d.a, d.b, d.c, R.rot_ch, R.rot_sh, qmv);
-            remc_qcoeffs(qmv, coef);
-        } else {
-            remc_move_centre(d.a, d.b, d.c, R.tstep, S.box, cx, cy, cz, &ncx, &ncy, &ncz);
-        }
+        if (!rotate) remc_wrap_centre(cx, cy, cz, D.t[0], D.t[1], D.t[2], S.box, &ncx, &ncy, &ncz);
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
             float rx = grx[b], ry = gry[b], rz = grz[b];
             float nrx = rx, nry = ry, nrz = rz;
-            if (d.rotate) remc_qrotate(coef, rx, ry, rz, &nrx, &nry, &nrz);
+            if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
             s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b]);
             s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
             s_tc[k] = tcs[b];
@@ -202,10 +235,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         // this member's share of dE = sum_{i in body} sum_{j in slice, not in body} e(new_i,j) - e(old_i,j),
         // split into tk groups of moved beads x G lanes over partner beads
         const int ov = max(0, min(je, m1) - max(js, m0));
-        const int tk = pick_kgroups<BLOCK>(nm, (je - js) - ov);
-        const int G = BLOCK / tk;
-        const int g = tid / G, l = tid - g * G;
-        const int k0 = (g * nm) / tk, k1 = ((g + 1) * nm) / tk;
+        const int lt = pick_kgroups_log2<BLOCK>(nm, (je - js) - ov);
+        const int lgw = (BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10) - lt;
+        const int G = 1 << lgw;
+        const int g = tid >> lgw, l = tid & (G - 1);
+        const int k0 = (g * nm) >> lt, k1 = ((g + 1) * nm) >> lt;
         float acc = 0.0f, comp = 0.0f;
         for (int j = js + l; j < je; j += G) {
             if (j >= m0 && j < m1) continue;
@@ -301,7 +335,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         ++n_split;
         }
 
-        const int decision = remc_metropolis(dE, T, d.u);
+        const int decision = remc_metropolis(dE, T, D.u);
         if (decision) {
             for (int k = tid; k < nm; k += BLOCK) {
                 const int b = m0 + k;
@@ -309,27 +343,28 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 gx[b] = n.x;
                 gy[b] = n.y;
                 gz[b] = n.z;
-                if (d.rotate) {
+                if (rotate) {
                     float nrx, nry, nrz;
-                    remc_qrotate(coef, grx[b], gry[b], grz[b], &nrx, &nry, &nrz);
+                    remc_qrotate(D.coef, grx[b], gry[b], grz[b], &nrx, &nry, &nrz);
                     grx[b] = nrx;
                     gry[b] = nry;
                     grz[b] = nrz;
                 }
             }
             if (tid == 0) {
-                if (d.rotate) {
-                    double qo[4] = {gq[4 * d.body + 0], gq[4 * d.body + 1], gq[4 * d.body + 2], gq[4 * d.body + 3]};
+                if (rotate) {
+                    double qo[4] = {gq[4 * body + 0], gq[4 * body + 1], gq[4 * body + 2], gq[4 * body + 3]};
+                    double qm[4] = {D.q[0], D.q[1], D.q[2], D.q[3]};
                     double qn[4];
-                    remc_qmul(qmv, qo, qn);
-                    gq[4 * d.body + 0] = qn[0];
-                    gq[4 * d.body + 1] = qn[1];
-                    gq[4 * d.body + 2] = qn[2];
-                    gq[4 * d.body + 3] = qn[3];
+                    remc_qmul(qm, qo, qn);
+                    gq[4 * body + 0] = qn[0];
+                    gq[4 * body + 1] = qn[1];
+                    gq[4 * body + 2] = qn[2];
+                    gq[4 * body + 3] = qn[3];
                 } else {
-                    gcx[d.body] = ncx;
-                    gcy[d.body] = ncy;
-                    gcz[d.body] = ncz;
+                    gcx[body] = ncx;
+                    gcy[body] = ncy;
+                    gcz[body] = ncz;
                 }
             }
             E += dE;
@@ -341,10 +376,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             unsigned long long idx = step - rec_start;
             if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
                 remc_step_record rr;
-                rr.body = d.body;
-                rr.move = d.rotate;
+                rr.body = body;
+                rr.move = rotate ? 1 : 0;
                 rr.accepted = decision;
-                rr.pad = 0;
+                rr.ticks = (int32_t)(wall_clock64() - t_step); // 100 MHz ticks spent on the step
                 rr.delta = dE;
                 p.rec[(size_t)r * p.rec_max + idx] = rr;
             }

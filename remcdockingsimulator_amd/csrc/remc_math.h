@@ -183,21 +183,36 @@ REMC_HD void remc_move_quat(double a, double b, double c, double ch, double sh, 
     q[0] = q[0] / qm; q[1] = q[1] / qm; q[2] = q[2] / qm; q[3] = q[3] / qm;
 }
 
-// New centre of a translate move (Replica::translate, Replica.cpp:333-364):
-// t = step * normalize((float)a,(float)b,(float)c) in FP32, wrapped with fmod.
-REMC_HD void remc_move_centre(double a, double b, double c, float step, float box, float cx, float cy, float cz,
-                              float *px, float *py, float *pz)
+// Translation vector of a translate move (Replica::translate,
+// Replica.cpp:333-337): t = step * normalize((float)a,(float)b,(float)c) in
+// FP32.  It depends on the draws only, so it can be formed ahead of the step.
+REMC_HD void remc_move_translation(double a, double b, double c, float step, float *tx, float *ty, float *tz)
 {
     float vx = (float)a, vy = (float)b, vz = (float)c;
     float mag = sqrtf(vx * vx + vy * vy + vz * vz);
     if (mag != 0.0f) { vx = vx / mag; vy = vy / mag; vz = vz / mag; }
-    float x = cx + vx * step, y = cy + vy * step, z = cz + vz * step;
+    *tx = vx * step; *ty = vy * step; *tz = vz * step;
+}
+
+// New centre: c + t wrapped with fmod(. + L, L) per axis (Replica.cpp:358-362).
+REMC_HD void remc_wrap_centre(float cx, float cy, float cz, float tx, float ty, float tz, float box,
+                              float *px, float *py, float *pz)
+{
+    float x = cx + tx, y = cy + ty, z = cz + tz;
     if (box > 0.0f) {
         x = remc_fmodf(x + box, box);
         y = remc_fmodf(y + box, box);
         z = remc_fmodf(z + box, box);
     }
     *px = x; *py = y; *pz = z;
+}
+
+REMC_HD void remc_move_centre(double a, double b, double c, float step, float box, float cx, float cy, float cz,
+                              float *px, float *py, float *pz)
+{
+    float tx, ty, tz;
+    remc_move_translation(a, b, c, step, &tx, &ty, &tz);
+    remc_wrap_centre(cx, cy, cz, tx, ty, tz, box, px, py, pz);
 }
 
 // Metropolis test in FP64 (Replica.cpp:438-459): 1 = accept (dE < 0),
