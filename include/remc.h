@@ -200,6 +200,10 @@ REMC_API int remc_get_mc_block(remc_ctx *ctx, int *block);
  * co-resident on the device (checked). */
 REMC_API int remc_set_mc_coop(remc_ctx *ctx, int coop);
 REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
+/* MC kernel tuning: lds_cache = 0 keeps partner beads in HBM instead of the
+ * LDS cache; split_min_pairs (> 0) is the pair-term count from which a move is
+ * split across cooperating workgroups (default 8192; 0 keeps the current). */
+REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
 
 #ifdef __cplusplus
 }

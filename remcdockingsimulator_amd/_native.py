@@ -82,6 +82,7 @@ SIGNATURES = {
     "remc_get_mc_block": (ctypes.c_int, [_vp, _ip]),
     "remc_set_mc_coop": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_get_mc_coop": (ctypes.c_int, [_vp, _ip]),
+    "remc_set_mc_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
 }
 
 _lib = None

@@ -121,6 +121,10 @@ struct remc_ctx {
     int coop_min_pairs = 8192; // moves below this many pair terms are evaluated whole by every member
     int synced_copies = 1;   // state copies [0, synced_copies) hold the current conformation
     int n_cu = 0;
+    int lds_per_block = 0, lds_per_cu = 0;
+    bool lds_cache = true;   // partner-bead LDS cache of the MC kernel allowed
+    long long min_move_pairs = -1;
+    int max_bodies = 0, max_body_beads = 0;
 
     // graph
     hipGraphExec_t graph = nullptr;
@@ -404,6 +408,23 @@ int finalize(remc_ctx *c)
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, c->device));
     c->n_cu = prop.multiProcessorCount;
+    c->lds_per_block = (int)prop.sharedMemPerBlock;
+    c->lds_per_cu = (int)prop.maxSharedMemoryPerMultiProcessor;
+    if (c->lds_per_cu <= 0) c->lds_per_cu = c->lds_per_block;
+    // move-size facts used to pick the LDS caches of the MC kernel
+    c->min_move_pairs = -1;
+    for (auto &sy : c->systems)
+        for (int m = 0; m < sy.n_bodies; ++m) {
+            const long long nm = sy.body_start[m + 1] - sy.body_start[m];
+            const long long pairs = nm * (sy.n_beads - nm);
+            if (c->min_move_pairs < 0 || pairs < c->min_move_pairs) c->min_move_pairs = pairs;
+        }
+    c->max_bodies = 0;
+    c->max_body_beads = 0;
+    for (auto &sy : c->systems) {
+        c->max_bodies = std::max(c->max_bodies, sy.n_bodies);
+        c->max_body_beads = std::max(c->max_body_beads, sy.max_body);
+    }
     HIP_TRY(hipStreamSynchronize(s));
     c->synced_copies = 1;
     c->finalized = true;
@@ -463,22 +484,65 @@ int check_err(remc_ctx *c)
     return REMC_OK;
 }
 
+// LDS layout of the MC kernel for a given cooperation: the moved-body staging
+// always; the partner beads of the replica (or of the member's slice when
+// every move is split) and the body table when they fit the LDS share of one
+// workgroup at the occupancy the grid needs.
+void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
+{
+    const int R = (int)c->reps.size();
+    const long long wgs = (long long)R * coop;
+    const int per_cu = (int)std::max<long long>(1, (wgs + c->n_cu - 1) / c->n_cu);
+    const int static_lds = 16 * 1024; // etab, draws, reduction scratch and slack
+    long long budget = std::min<long long>(c->lds_per_block, c->lds_per_cu / per_cu) - static_lds;
+    p.lds_mb = std::max(1, c->max_body_beads);
+    budget -= 36LL * p.lds_mb;
+    p.lds_cl = 0;
+    p.lds_nbody = 0;
+    p.cache_slice = 0;
+    p.cache_bodies = 0;
+    if (!c->lds_cache || budget <= 0) return;
+    const long long full = 24LL * c->max_beads;
+    const bool all_split = coop > 1 && c->min_move_pairs >= c->coop_min_pairs;
+    const long long slice = 24LL * ((c->max_beads + coop - 1) / coop + 1);
+    if (full <= budget) {
+        p.lds_cl = c->max_beads;
+        budget -= full;
+    } else if (all_split && slice <= budget) {
+        p.lds_cl = (c->max_beads + coop - 1) / coop + 1;
+        p.cache_slice = 1;
+        budget -= slice;
+    } else {
+        return;
+    }
+    if (20LL * c->max_bodies <= budget) {
+        p.lds_nbody = c->max_bodies;
+        p.cache_bodies = 1;
+    }
+}
+
 // Largest workgroups-per-replica count whose grid is co-resident.
 int max_coop(remc_ctx *c, int block, int *out)
 {
-    int per_cu = 0;
-    HIP_TRY(mc_max_active_per_cu(block, &per_cu));
     const int R = (int)c->reps.size();
     int m = REMC_MAX_COOP;
-    while (m > 1 && (long long)m * R > (long long)per_cu * c->n_cu) m >>= 1;
+    for (; m > 1; m >>= 1) {
+        KParams p = c->params();
+        fill_mc_lds(c, p, m);
+        int per_cu = 0;
+        HIP_TRY(mc_max_active_per_cu(p, block, &per_cu));
+        if ((long long)m * R <= (long long)per_cu * c->n_cu) break;
+    }
     *out = m;
     return REMC_OK;
 }
 
-int mc_launch(remc_ctx *c, const KParams &p, int nsteps, int block, int coop)
+int mc_launch(remc_ctx *c, const KParams &p0, int nsteps, int block, int coop)
 {
     int rc = sync_copies(c, coop);
     if (rc != REMC_OK) return rc;
+    KParams p = p0;
+    fill_mc_lds(c, p, coop);
     HIP_TRY(launch_mc(p, (int)c->reps.size(), nsteps, block, coop, c->stream));
     c->synced_copies = coop;
     return REMC_OK;
@@ -963,6 +1027,9 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
     if (!c->graph || c->graph_steps != steps_per_round) {
         destroy_graph(c);
         KParams p = c->params();
+        fill_mc_lds(c, p, c->mc_coop);
+        int per_cu = 0; // also sets the kernel's dynamic-LDS attribute before capture
+        HIP_TRY(mc_max_active_per_cu(p, c->mc_block, &per_cu));
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e1 = launch_mc(p, (int)c->reps.size(), steps_per_round, c->mc_block, c->mc_coop, c->stream);
@@ -1195,6 +1262,16 @@ int remc_get_mc_coop(remc_ctx *c, int *coop)
     CHECK_CTX(c);
     if (!coop) return fail(REMC_ERR_ARG, "null output");
     *coop = c->mc_coop;
+    return REMC_OK;
+}
+
+int remc_set_mc_tuning(remc_ctx *c, int lds_cache, int split_min_pairs)
+{
+    CHECK_CTX(c);
+    if (split_min_pairs < 0) return fail(REMC_ERR_ARG, "split threshold must be >= 0");
+    c->lds_cache = lds_cache != 0;
+    if (split_min_pairs > 0) c->coop_min_pairs = split_min_pairs;
+    destroy_graph(c);
     return REMC_OK;
 }
 

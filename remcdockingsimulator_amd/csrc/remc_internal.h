@@ -80,6 +80,12 @@ struct KParams {
     size_t copy_bodies;     // stride between state copies (bodies)
     unsigned long long *slots; // n_rep * 2 * REMC_MAX_COOP
     int *err;               // set when a member waited too long
+    // MC dynamic LDS: moved-body staging size and partner-bead/body caches
+    int lds_mb;             // largest body (beads)
+    int lds_cl;             // cached partner beads per workgroup (0 = no cache)
+    int lds_nbody;          // cached bodies per workgroup (0 = none)
+    int cache_slice;        // 1: each member caches only its partner slice
+    int cache_bodies;       // 1: body ranges and centres cached
     // recording
     remc_step_record *rec;
     unsigned long long *rec_start;
@@ -89,7 +95,7 @@ struct KParams {
 
 // launchers (remc_kernels.hip)
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
-hipError_t mc_max_active_per_cu(int block, int *out);
+hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
                               int noncrowder_only, int write_E, hipStream_t s);
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);

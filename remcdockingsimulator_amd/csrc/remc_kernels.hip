@@ -140,18 +140,54 @@ struct StepDraw {
 };
 constexpr int DRAW_BATCH = 64;
 
-template <int BLOCK>
+// 64-bit DPP move of a double (both halves with the same control word).
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int l2 = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, true);
+    const int h2 = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(h2, l2);
+}
+
+// Sum over the 64 lanes of a wave in a fixed order, with DPP row shifts and
+// row broadcasts (no LDS traffic); the result is uniform.
+__device__ __forceinline__ double wave_sum_dpp(double v)
+{
+    v += dpp_f64<0x111, 0xf>(v); // row_shr:1
+    v += dpp_f64<0x112, 0xf>(v); // row_shr:2
+    v += dpp_f64<0x114, 0xf>(v); // row_shr:4
+    v += dpp_f64<0x118, 0xf>(v); // row_shr:8: lane 15 of each row holds the row sum
+    v += dpp_f64<0x142, 0xa>(v); // row_bcast:15: rows 1 and 3 add the previous row's sum
+    v += dpp_f64<0x143, 0xc>(v); // row_bcast:31: rows 2 and 3 add lane 31
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), 63);
+    return __hiloint2double(hi, lo);
+}
+
+// Dynamic LDS of the MC kernel, carved in this order:
+//   float4 old[mb], float4 new[mb], int tc[mb]           moved body staging
+//   CACHE: float x,y,z,qdh,sig[cl], int tc[cl]            partner beads [cache_lo, cache_lo+cl)
+//   CACHE && cache_bodies: int start,end[nbody]; float cx,cy,cz[nbody]
+// cl, nbody and mb are the launch-wide maxima (KParams::lds_*).
+template <int BLOCK, bool CACHE>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
-    __shared__ float4 s_old[REMC_MAX_BODY_BEADS]; // x,y,z,q   current positions of the moved body
-    __shared__ float4 s_new[REMC_MAX_BODY_BEADS]; // x,y,z,sig proposed positions
-    __shared__ int s_tc[REMC_MAX_BODY_BEADS];
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
     __shared__ StepDraw s_draw[DRAW_BATCH];
+    extern __shared__ float4 smem[];
+    float4 *s_old = smem;
+    float4 *s_new = s_old + p.lds_mb;
+    int *s_tc = (int *)(s_new + p.lds_mb);
+    float *s_jx = (float *)(s_tc + p.lds_mb);
+    float *s_jy = s_jx + p.lds_cl, *s_jz = s_jy + p.lds_cl, *s_jq = s_jz + p.lds_cl, *s_js = s_jq + p.lds_cl;
+    int *s_jtc = (int *)(s_js + p.lds_cl);
+    int *s_bs = s_jtc + p.lds_cl, *s_be = s_bs + p.lds_nbody;
+    float *s_cx = (float *)(s_be + p.lds_nbody), *s_cy = s_cx + p.lds_nbody, *s_cz = s_cy + p.lds_nbody;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -170,19 +206,44 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
     const unsigned long long rec_start = p.rec ? p.rec_start[r] : 0ull;
 
-    // private copy c of this replica's dynamic state
+    // private copy c of this replica's dynamic state (always current in HBM)
     const size_t cb = (size_t)c * p.copy_beads + R.bead_off;
-    const size_t cm = (size_t)c * p.copy_bodies + R.body_off;
+    const size_t cmo = (size_t)c * p.copy_bodies + R.body_off;
     float *gx = p.px + cb, *gy = p.py + cb, *gz = p.pz + cb;
     float *grx = p.rx + cb, *gry = p.ry + cb, *grz = p.rz + cb;
-    float *gcx = p.cx + cm, *gcy = p.cy + cm, *gcz = p.cz + cm;
-    double *gq = p.quat + 4 * cm;
+    float *gcx = p.cx + cmo, *gcy = p.cy + cmo, *gcz = p.cz + cmo;
+    double *gq = p.quat + 4 * cmo;
     const int *tcs = p.tc + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
     const int cjs = (int)(((long long)c * N) / coop), cje = (int)(((long long)(c + 1) * N) / coop);
     unsigned long long *slots = p.slots + (size_t)r * 2 * REMC_MAX_COOP;
     unsigned int n_split = 0; // split steps so far in this launch: selects the slot parity
+
+    // cached partner beads: all of them, or only this member's slice when
+    // every move of the launch is split (cache_slice)
+    const int clo = (CACHE && p.cache_slice) ? cjs : 0;
+    const int chi = CACHE ? ((p.cache_slice) ? cje : N) : 0;
+    const bool cache_bodies = CACHE && p.cache_bodies;
+    if (CACHE) {
+        for (int b = clo + tid; b < chi; b += BLOCK) {
+            const int k = b - clo;
+            s_jx[k] = gx[b];
+            s_jy[k] = gy[b];
+            s_jz[k] = gz[b];
+            s_jq[k] = qs[b] * ff.c_dh;
+            s_js[k] = sgs[b];
+            s_jtc[k] = tcs[b];
+        }
+        if (cache_bodies)
+            for (int m = tid; m < S.n_bodies; m += BLOCK) {
+                s_bs[m] = p.bstart[S.body_off + m];
+                s_be[m] = p.bend[S.body_off + m];
+                s_cx[m] = gcx[m];
+                s_cy[m] = gcy[m];
+                s_cz[m] = gcz[m];
+            }
+    }
     __syncthreads();
 
     for (int it = 0; it < nsteps; ++it, ++step) {
@@ -211,10 +272,24 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         const StepDraw &D = s_draw[slot_d];
         const int body = D.body;
         const bool rotate = D.rotate != 0;
-        const int gb = S.body_off + body;
-        const int m0 = p.bstart[gb], m1 = p.bend[gb], nm = m1 - m0;
-        const bool crow_m = p.bcrow[gb] != 0;
-        const float cx = gcx[body], cy = gcy[body], cz = gcz[body];
+        int m0, m1;
+        float cx, cy, cz;
+        if (cache_bodies) {
+            m0 = s_bs[body];
+            m1 = s_be[body];
+            cx = s_cx[body];
+            cy = s_cy[body];
+            cz = s_cz[body];
+        } else {
+            const int gb = S.body_off + body;
+            m0 = p.bstart[gb];
+            m1 = p.bend[gb];
+            cx = gcx[body];
+            cy = gcy[body];
+            cz = gcz[body];
+        }
+        const int nm = m1 - m0;
+        const bool crow_m = p.bcrow[S.body_off + body] != 0;
         float ncx = cx, ncy = cy, ncz = cz;
         if (!rotate) remc_wrap_centre(cx, cy, cz, D.t[0], D.t[1], D.t[2], S.box, &ncx, &ncy, &ncz);
         for (int k = tid; k < nm; k += BLOCK) {
@@ -222,7 +297,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             float rx = grx[b], ry = gry[b], rz = grz[b];
             float nrx = rx, nry = ry, nrz = rz;
             if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
-            s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b]);
+            const bool inc = CACHE && b >= clo && b < chi;
+            s_old[k] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qs[b])
+                           : make_float4(gx[b], gy[b], gz[b], qs[b]);
             s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
             s_tc[k] = tcs[b];
         }
@@ -243,8 +320,24 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         float acc = 0.0f, comp = 0.0f;
         for (int j = js + l; j < je; j += G) {
             if (j >= m0 && j < m1) continue;
-            const float xj = gx[j], yj = gy[j], zj = gz[j];
-            const int tcj = tcs[j];
+            float xj, yj, zj, qj, sj;
+            int tcj;
+            if (CACHE) {
+                const int kj = j - clo;
+                xj = s_jx[kj];
+                yj = s_jy[kj];
+                zj = s_jz[kj];
+                qj = s_jq[kj];
+                sj = s_js[kj];
+                tcj = s_jtc[kj];
+            } else {
+                xj = gx[j];
+                yj = gy[j];
+                zj = gz[j];
+                qj = qs[j] * ff.c_dh;
+                sj = sgs[j];
+                tcj = tcs[j];
+            }
             const bool cj = (tcj >> 8) != 0;
             if (crow_m || cj) {
                 for (int k = k0; k < k1; ++k) {
@@ -260,7 +353,6 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     kahan_add(acc, comp, en - eo);
                 }
             } else {
-                const float qj = qs[j] * ff.c_dh, sj = sgs[j];
                 const float *erow = s_etab + (tcj & 0xff);
                 for (int k = k0; k < k1; ++k) {
                     const float4 o = s_old[k], n = s_new[k];
@@ -281,7 +373,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
             }
         }
-        double v = wave_sum((double)acc - (double)comp);
+        const double v = wave_sum_dpp((double)acc - (double)comp);
         if (lane == 0) s_red[wave] = v;
         __syncthreads();
         double dE = 0.0;
@@ -289,11 +381,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 #pragma unroll
             for (int w = 0; w < NW; ++w) dE += s_red[w];
         } else {
-        if (wave == 0) {
-            double part = 0.0;
+            if (wave == 0) {
+                double part = 0.0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) part += s_red[w];
-            {
+                for (int w = 0; w < NW; ++w) part += s_red[w];
                 // slots alternate with the split-step count: a member reuses a
                 // slot only after every member has read it (they all wrote
                 // their partial of the split step in between)
@@ -309,7 +400,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         w = __hip_atomic_load(slot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                         ok = (w & SLOT_TAG_MASK) == tag;
                     }
-                    if (++spins >= SLOT_SPIN_LIMIT) { failed = true; break; }
+                    if (++spins >= SLOT_SPIN_LIMIT) {
+                        failed = true;
+                        break;
+                    }
                     if ((spins & 255) == 0 && __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                         failed = true;
                         break;
@@ -328,11 +422,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     if (lane == 0) s_dE = tot;
                 }
             }
-        }
-        __syncthreads();
-        if (s_abort) break;
-        dE = s_dE;
-        ++n_split;
+            __syncthreads();
+            if (s_abort) break;
+            dE = s_dE;
+            ++n_split;
         }
 
         const int decision = remc_metropolis(dE, T, D.u);
@@ -343,6 +436,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 gx[b] = n.x;
                 gy[b] = n.y;
                 gz[b] = n.z;
+                if (CACHE && b >= clo && b < chi) {
+                    s_jx[b - clo] = n.x;
+                    s_jy[b - clo] = n.y;
+                    s_jz[b - clo] = n.z;
+                }
                 if (rotate) {
                     float nrx, nry, nrz;
                     remc_qrotate(D.coef, grx[b], gry[b], grz[b], &nrx, &nry, &nrz);
@@ -365,6 +463,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     gcx[body] = ncx;
                     gcy[body] = ncy;
                     gcz[body] = ncz;
+                    if (cache_bodies) {
+                        s_cx[body] = ncx;
+                        s_cy[body] = ncy;
+                        s_cz[body] = ncz;
+                    }
                 }
             }
             E += dE;
@@ -401,6 +504,22 @@ __global__ void k_reset_slots(unsigned long long *slots, int n)
     if (i < n) slots[i] = ~0ull;
 }
 
+size_t mc_dynamic_lds(const KParams &p)
+{
+    return sizeof(float4) * 2 * (size_t)p.lds_mb + sizeof(int) * (size_t)p.lds_mb +
+           (sizeof(float) * 5 + sizeof(int)) * (size_t)p.lds_cl + (sizeof(int) * 2 + sizeof(float) * 3) * (size_t)p.lds_nbody;
+}
+
+template <int BLOCK, bool CACHE>
+static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, hipStream_t s)
+{
+    const size_t lds = mc_dynamic_lds(p);
+    hipError_t e = hipFuncSetAttribute((const void *)k_mc<BLOCK, CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_mc<BLOCK, CACHE>), dim3(grid), dim3(BLOCK), lds, s, p, nsteps, coop);
+    return hipGetLastError();
+}
+
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s)
 {
     if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
@@ -409,24 +528,35 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
         const int n = n_rep * 2 * REMC_MAX_COOP;
         hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
     }
-    const dim3 grid(n_rep * coop);
+    const int grid = n_rep * coop;
+    const bool cache = p.lds_cl > 0;
     switch (block) {
-    case 256: hipLaunchKernelGGL(k_mc<256>, grid, dim3(256), 0, s, p, nsteps, coop); break;
-    case 512: hipLaunchKernelGGL(k_mc<512>, grid, dim3(512), 0, s, p, nsteps, coop); break;
-    case 1024: hipLaunchKernelGGL(k_mc<1024>, grid, dim3(1024), 0, s, p, nsteps, coop); break;
+    case 256: return cache ? launch_mc_t<256, true>(p, grid, nsteps, coop, s) : launch_mc_t<256, false>(p, grid, nsteps, coop, s);
+    case 512: return cache ? launch_mc_t<512, true>(p, grid, nsteps, coop, s) : launch_mc_t<512, false>(p, grid, nsteps, coop, s);
+    case 1024: return cache ? launch_mc_t<1024, true>(p, grid, nsteps, coop, s) : launch_mc_t<1024, false>(p, grid, nsteps, coop, s);
     default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
-hipError_t mc_max_active_per_cu(int block, int *out)
+hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out)
 {
+    const size_t lds = mc_dynamic_lds(p);
+    const bool cache = p.lds_cl > 0;
+#define REMC_OCC(B)                                                                                                    \
+    do {                                                                                                               \
+        const void *f = cache ? (const void *)k_mc<B, true> : (const void *)k_mc<B, false>;                            \
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                   \
+        if (e != hipSuccess) return e;                                                                                 \
+        return cache ? hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<B, true>, B, lds)                        \
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<B, false>, B, lds);                      \
+    } while (0)
     switch (block) {
-    case 256: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<256>, 256, 0);
-    case 512: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<512>, 512, 0);
-    case 1024: return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<1024>, 1024, 0);
+    case 256: REMC_OCC(256);
+    case 512: REMC_OCC(512);
+    case 1024: REMC_OCC(1024);
     default: return hipErrorInvalidValue;
     }
+#undef REMC_OCC
 }
 
 // --------------------------------------------------------------------------

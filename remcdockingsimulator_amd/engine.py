@@ -161,6 +161,10 @@ class Engine:
         self._chk(self.lib.remc_get_mc_coop(self.ctx, ctypes.byref(b)), "remc_get_mc_coop")
         return b.value
 
+    def set_mc_tuning(self, lds_cache=True, split_min_pairs=0):
+        self._chk(self.lib.remc_set_mc_tuning(self.ctx, 1 if lds_cache else 0, int(split_min_pairs)),
+                  "remc_set_mc_tuning")
+
     def adjoint(self, replica):
         m = self.systems[self.replica_sys[replica]].n_bodies
         ft = np.zeros((m, 6), dtype=np.float64)

@@ -81,6 +81,48 @@ def test_mc_cooperative_workgroups(oracle, coop):
     assert run_parity(oracle, 1, 500, block=256, coop=coop) <= 1
 
 
+@pytest.mark.parametrize("lds_cache", [False, True])
+def test_mc_partner_cache_modes(oracle, lds_cache):
+    """Partner beads read from HBM or from the LDS cache: same decisions."""
+    wl = synth.config(2)
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED, block=512)
+    e.set_mc_coop(2)
+    e.set_mc_tuning(lds_cache=lds_cache)
+    e.set_recording(400)
+    e.mc_steps(400)
+    e.synchronize()
+    o = oracle.OracleSystem(sa, wl.ff)
+    for r in (0, 19):
+        orep = oracle.OracleReplica(o, lad[r], wl.translate_step, wl.rotate_step, r)
+        n_eq, div = compare_decisions(e.records(r), orep.mc(seed=SEED, nsteps=400, mode=0), lad[r])
+        if not div:
+            assert np.array_equal(e.state(r)["pos"], orep.pos)
+    e.close()
+
+
+def test_mc_config5_member_slices(oracle):
+    """Config 5 (11000 beads): every move is split over 4 members, each caching
+    only its partner slice."""
+    wl = synth.config(5)
+    sa = wl.systems[0]
+    lad = wl.temperatures[0]
+    e = engine_for(wl, seed=SEED, block=512)
+    e.set_mc_coop(4)
+    e.set_recording(60)
+    e.mc_steps(60)
+    e.synchronize()
+    o = oracle.OracleSystem(sa, wl.ff)
+    for r in (0, 63):
+        say("config 5 replica", r)
+        orep = oracle.OracleReplica(o, lad[r], wl.translate_step, wl.rotate_step, r)
+        n_eq, div = compare_decisions(e.records(r), orep.mc(seed=SEED, nsteps=60, mode=0), lad[r])
+        if not div:
+            assert np.array_equal(e.state(r)["pos"], orep.pos)
+    e.close()
+
+
 def test_cooperative_state_resync_after_host_changes(oracle):
     """Plain (host-protocol) steps and set_state change copy 0 only; the next
     cooperative launch must refresh the other copies first."""
