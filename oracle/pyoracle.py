@@ -109,6 +109,10 @@ def lib():
         L.orc_step_words.argtypes = [ctypes.c_uint64, ctypes.c_uint32, _u32p, _u32p]
         L.orc_philox4x32_10.restype = None
         L.orc_philox4x32_10.argtypes = [_u32p, _u32p, _u32p]
+        L.orc_energy_full_d.restype = ctypes.c_double
+        L.orc_energy_full_d.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _dp, _dp, _dp]
+        L.orc_gradient.restype = None
+        L.orc_gradient.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield)] + [_dp] * 9
         _lib = L
     return _lib
 
@@ -181,6 +185,25 @@ class OracleSystem:
         return self.L.orc_delta_energy(ctypes.byref(self.s), ctypes.byref(self.ff), x.ctypes.data_as(_fp),
                                        y.ctypes.data_as(_fp), z.ctypes.data_as(_fp), m, nx.ctypes.data_as(_fp),
                                        ny.ctypes.data_as(_fp), nz.ctypes.data_as(_fp))
+
+    def energy_d(self, pos, ff=None):
+        """Full energy from float64 positions (optionally another force field)."""
+        pos = np.asarray(pos, dtype=np.float64)
+        x, y, z = (np.ascontiguousarray(pos[:, k]) for k in range(3))
+        return self.L.orc_energy_full_d(ctypes.byref(self.s), ctypes.byref(self.ff if ff is None else ff),
+                                        x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), z.ctypes.data_as(_dp))
+
+    def gradient(self, pos, rel):
+        """(bead dE/dx (n,3), per-body force+torque (m,6), dE/d(lambda,e0,xi))."""
+        pos = np.asarray(pos, dtype=np.float64)
+        rel = np.asarray(rel, dtype=np.float64)
+        cols = [np.ascontiguousarray(pos[:, k]) for k in range(3)] + [np.ascontiguousarray(rel[:, k]) for k in range(3)]
+        g = np.zeros(3 * self.sa.n_beads)
+        ft = np.zeros(6 * self.sa.n_bodies)
+        dp = np.zeros(3)
+        self.L.orc_gradient(ctypes.byref(self.s), ctypes.byref(self.ff), *[c.ctypes.data_as(_dp) for c in cols],
+                            g.ctypes.data_as(_dp), ft.ctypes.data_as(_dp), dp.ctypes.data_as(_dp))
+        return g.reshape(-1, 3), ft.reshape(-1, 6), dp
 
     def pair_table(self, pos, a, b):
         x, y, z = self._xyz(pos)

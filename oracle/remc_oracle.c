@@ -290,6 +290,103 @@ double orc_delta_energy(const orc_system *s, const orc_forcefield *ff,
     return acc;
 }
 
+/* ------------------------------------------------------------ adjoint ---- */
+double orc_energy_full_d(const orc_system *s, const orc_forcefield *ff,
+                         const double *x, const double *y, const double *z)
+{
+    double e = 0.0;
+    for (int a = 0; a < s->n_bodies; ++a)
+        for (int b = a + 1; b < s->n_bodies; ++b)
+            for (int i = s->body_start[a]; i < s->body_start[a + 1]; ++i)
+                for (int j = s->body_start[b]; j < s->body_start[b + 1]; ++j)
+                    e += orc_pair_ij(s, ff, x[i], y[i], z[i], i, s->crowder[a], x[j], y[j], z[j], j,
+                                     s->crowder[b], NULL, NULL);
+    return e;
+}
+
+/* Pair energy derivatives at distance r (the same branches as
+ * orc_pair_energy): de/dr, de/dE_ij and de/dxi, all in kcal/mol units. */
+static void orc_pair_derivs(const orc_forcefield *ff, double r, int ti, double qi, double si, int ci,
+                            int tj, double qj, double sj, int cj, double *dedr, double *de_deij, double *de_dxi)
+{
+    const double c_lj = ff->lj_cf * ff->kbt_cf, c_dh = ff->dh_cf * ff->kbt_cf;
+    double dlj = 0.0, ddh = 0.0, dlj_de = 0.0, ddh_dxi = 0.0;
+    if (ci || cj) {
+        if (r < ff->crowder_cutoff) {
+            double t = ff->crowder_sigma / r, t2 = t * t, t6 = t2 * t2 * t2;
+            dlj = -6.0 * t6 / r;
+        }
+    } else {
+        double dh = qi * qj * exp(-r / ff->xi) / r;
+        ddh = -dh * (1.0 / ff->xi + 1.0 / r);
+        ddh_dxi = dh * r / (ff->xi * ff->xi);
+        double eij = ff->lambda * (ff->table[ti * ORC_NTYPES + tj] - ff->e0);
+        double sij = 0.5 * (si + sj);
+        double t = sij / r, t2 = t * t, x6 = t2 * t2 * t2;
+        double base = 4.0 * x6 * (x6 - 1.0);
+        double dbase = 4.0 * (2.0 * x6 - 1.0) * (-6.0 * x6 / r);
+        if (eij > 0.0 && r < sij * 1.122462048309372981433533049679) {
+            dlj = eij * dbase;          /* LJ = E (base + 2) */
+            dlj_de = base + 2.0;
+        } else {
+            dlj = -eij * dbase;         /* LJ = -E base */
+            dlj_de = -base;
+        }
+    }
+    *dedr = dlj * c_lj + ddh * c_dh;
+    *de_deij = dlj_de * c_lj;
+    *de_dxi = ddh_dxi * c_dh;
+}
+
+void orc_gradient(const orc_system *s, const orc_forcefield *ff,
+                  const double *x, const double *y, const double *z,
+                  const double *rx, const double *ry, const double *rz,
+                  double *bead_grad, double *force_torque, double *dparams)
+{
+    const int n = s->n_beads;
+    double *g = (double *)calloc((size_t)3 * n, sizeof(double));
+    double dl = 0.0, de0 = 0.0, dxi = 0.0;
+    for (int a = 0; a < s->n_bodies; ++a)
+        for (int b = a + 1; b < s->n_bodies; ++b)
+            for (int i = s->body_start[a]; i < s->body_start[a + 1]; ++i)
+                for (int j = s->body_start[b]; j < s->body_start[b + 1]; ++j) {
+                    double dx = orc_min_image(x[i] - x[j], s->box);
+                    double dy = orc_min_image(y[i] - y[j], s->box);
+                    double dz = orc_min_image(z[i] - z[j], s->box);
+                    double d = sqrt(dx * dx + dy * dy + dz * dz);
+                    double r = d + ff->eps_r;
+                    double dedr, de_deij, de_dxi;
+                    orc_pair_derivs(ff, r, s->type[i], s->charge[i], s->sigma[i], s->crowder[a], s->type[j],
+                                    s->charge[j], s->sigma[j], s->crowder[b], &dedr, &de_deij, &de_dxi);
+                    double k = d > 0.0 ? dedr / d : 0.0;
+                    g[3 * i + 0] += k * dx; g[3 * i + 1] += k * dy; g[3 * i + 2] += k * dz;
+                    g[3 * j + 0] -= k * dx; g[3 * j + 1] -= k * dy; g[3 * j + 2] -= k * dz;
+                    if (!(s->crowder[a] || s->crowder[b])) {
+                        dl += de_deij * (ff->table[s->type[i] * ORC_NTYPES + s->type[j]] - ff->e0);
+                        de0 += de_deij * (-ff->lambda);
+                        dxi += de_dxi;
+                    }
+                }
+    if (bead_grad) memcpy(bead_grad, g, sizeof(double) * 3 * (size_t)n);
+    if (force_torque)
+        for (int m = 0; m < s->n_bodies; ++m) {
+            double F[3] = {0, 0, 0}, T[3] = {0, 0, 0};
+            for (int i = s->body_start[m]; i < s->body_start[m + 1]; ++i) {
+                double f[3] = {-g[3 * i], -g[3 * i + 1], -g[3 * i + 2]};
+                F[0] += f[0]; F[1] += f[1]; F[2] += f[2];
+                T[0] += ry[i] * f[2] - rz[i] * f[1];
+                T[1] += rz[i] * f[0] - rx[i] * f[2];
+                T[2] += rx[i] * f[1] - ry[i] * f[0];
+            }
+            for (int k = 0; k < 3; ++k) {
+                force_torque[6 * m + k] = F[k];
+                force_torque[6 * m + 3 + k] = T[k];
+            }
+        }
+    if (dparams) { dparams[0] = dl; dparams[1] = de0; dparams[2] = dxi; }
+    free(g);
+}
+
 /* -------------------------------------------------------------- moves ---- */
 /* Quaternion::multiply (Quaternion.cpp:35-47): out = a * b, (w,x,y,z). */
 void orc_quat_mul(const double a[4], const double b[4], double out[4])

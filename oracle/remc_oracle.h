@@ -139,6 +139,19 @@ double orc_delta_energy(const orc_system *s, const orc_forcefield *ff,
 void orc_pair_table(const orc_system *s, const orc_forcefield *ff,
                     const float *x, const float *y, const float *z, int a, int b, double *out);
 
+/* adjoint ------------------------------------------------------------------ */
+/* Full energy from FP64 positions (for finite differences). */
+double orc_energy_full_d(const orc_system *s, const orc_forcefield *ff,
+                         const double *x, const double *y, const double *z);
+/* Analytic gradient of the full energy: per-bead dE/dx (3*n_beads, may be NULL),
+ * per-body force and torque about the body centre (6*n_bodies: Fx Fy Fz Tx Ty
+ * Tz, F = -dE/dx summed over the body, T = sum rel x F), and dE/d(lambda, e0,
+ * xi).  rel are the bead offsets from their body centre. */
+void orc_gradient(const orc_system *s, const orc_forcefield *ff,
+                  const double *x, const double *y, const double *z,
+                  const double *rx, const double *ry, const double *rz,
+                  double *bead_grad, double *force_torque, double *dparams);
+
 /* moves -------------------------------------------------------------------- */
 void orc_quat_mul(const double a[4], const double b[4], double out[4]);
 void orc_quat_rotate(const double q[4], const float v[3], float out[3]);

@@ -113,6 +113,7 @@ struct remc_ctx {
     int rec_max = 0;
     unsigned long long *d_slots = nullptr;
     int *d_err = nullptr;
+    double *d_adj = nullptr; // adjoint scratch: bead gradients, tile partials, body forces, parameter gradient
     DevFF devff{};
 
     // MC configuration: workgroup size and workgroups per replica (0 = autotune)
@@ -666,7 +667,7 @@ int remc_ctx_destroy(remc_ctx *c)
                     c->d_sys, c->d_rep, c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz,
                     c->d_cx, c->d_cy, c->d_cz, c->d_quat, c->d_E, c->d_T, c->d_step, c->d_cnt,
                     c->d_slot, c->d_exoff, c->d_ladder, c->d_exround, c->d_exswaps, c->d_full_partials,
-                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err};
+                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err, c->d_adj};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->free_events) hipEventDestroy(e);
@@ -1201,8 +1202,29 @@ int remc_set_state(remc_ctx *c, int replica, const float *x, const float *y, con
 int remc_adjoint(remc_ctx *c, int replica, double *force_torque, double *dparams)
 {
     CHECK_CTX(c);
-    (void)replica; (void)force_torque; (void)dparams;
-    return fail(REMC_ERR_STATE, "adjoint kernel not built in this library version");
+    FINALIZE(c);
+    CHECK_REP(c, replica);
+    if (!force_torque && !dparams) return fail(REMC_ERR_ARG, "null outputs");
+    const HostSys &s = c->systems[c->reps[replica].sys];
+    const int tiles = (s.n_beads + FULL_TILE - 1) / FULL_TILE;
+    if (!c->d_adj) {
+        const size_t nd = 3 * (size_t)c->max_beads + 3 * (size_t)c->max_tiles + 6 * (size_t)c->max_bodies + 3;
+        HIP_TRY(hipMalloc(&c->d_adj, nd * sizeof(double)));
+    }
+    double *grad = c->d_adj, *part = grad + 3 * (size_t)c->max_beads, *ft = part + 3 * (size_t)c->max_tiles,
+           *dp = ft + 6 * (size_t)c->max_bodies;
+    std::vector<unsigned char> af(adjoint_ff_size());
+    adjoint_ff_fill(af.data(), c->ff.table, c->ff.lambda, c->ff.e0, c->ff.xi, c->ff.lj_cf * c->ff.kbt_cf,
+                    c->ff.dh_cf * c->ff.kbt_cf);
+    {
+        TimedLaunch t(c, 4);
+        HIP_TRY(launch_adjoint(c->params(), af.data(), replica, s.n_beads, s.n_bodies, grad, part, ft, dp, c->stream));
+    }
+    if (force_torque)
+        HIP_TRY(hipMemcpyAsync(force_torque, ft, sizeof(double) * 6 * s.n_bodies, hipMemcpyDeviceToHost, c->stream));
+    if (dparams) HIP_TRY(hipMemcpyAsync(dparams, dp, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
 }
 
 int remc_timing_enable(remc_ctx *c, int enable)

@@ -100,8 +100,10 @@ hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double
                               int noncrowder_only, int write_E, hipStream_t s);
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
 hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
-hipError_t launch_adjoint(const KParams &p, int replica, int n_beads, int n_bodies, double *bead_grad,
-                          double *param_grad, hipStream_t s);
+hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
+                          double *bead_grad, double *param_part, double *force_torque, double *dparams, hipStream_t s);
+size_t adjoint_ff_size();
+void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh);
 
 constexpr int FULL_TILE = 256;
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel

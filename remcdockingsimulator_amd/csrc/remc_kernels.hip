@@ -776,4 +776,173 @@ hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
     return hipGetLastError();
 }
 
+// --------------------------------------------------------------------------
+// adjoint: gradient of the full energy of one replica
+//
+// k_adjoint_beads: one thread per bead i, partner beads staged in 256-bead
+// LDS tiles; FP32 pair derivatives (same branches as the pair term),
+// FP64 accumulation of dE/dx_i and of the parameter derivatives
+// dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij,
+// dE/dxi = sum DH r / xi^2.  k_adjoint_bodies: per body F = -sum dE/dx_i and
+// torque sum rel_i x F_i about the body centre, and the parameter totals.
+// --------------------------------------------------------------------------
+struct AdjFF {
+    float table[REMC_NTYPES * REMC_NTYPES]; // raw table T_ij
+    float lambda, e0, xi, c_lj, c_dh;
+};
+
+__global__ __launch_bounds__(FULL_TILE) void k_adjoint_beads(KParams p, AdjFF af, int replica, double *bead_grad,
+                                                             double *param_part)
+{
+    __shared__ float s_tab[REMC_NTYPES * REMC_NTYPES];
+    __shared__ float s_x[FULL_TILE], s_y[FULL_TILE], s_z[FULL_TILE], s_q[FULL_TILE], s_s[FULL_TILE];
+    __shared__ int s_tc[FULL_TILE], s_body[FULL_TILE];
+    __shared__ double s_red[3][FULL_TILE / 64];
+    const int tid = threadIdx.x;
+    const DevRep R = p.rep[replica];
+    const DevSys S = p.sys[R.sys];
+    const DevFF ff = p.ff;
+    const int N = S.n_beads;
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += FULL_TILE) s_tab[k] = af.table[k];
+    const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
+    const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
+    const int i = blockIdx.x * FULL_TILE + tid;
+    const bool vi = i < N;
+    const float xi = vi ? gx[i] : 0.f, yi = vi ? gy[i] : 0.f, zi = vi ? gz[i] : 0.f;
+    const int tci = vi ? tcs[i] : 0, bi = vi ? bodies[i] : -1;
+    const float qi = vi ? qs[i] : 0.f, si = vi ? sgs[i] : 0.f;
+    const bool ci = (tci >> 8) != 0;
+    double gxa = 0.0, gya = 0.0, gza = 0.0, dl = 0.0, de0 = 0.0, dxi = 0.0;
+    const float inv_xi2 = 1.0f / (af.xi * af.xi);
+    for (int j0 = 0; j0 < N; j0 += FULL_TILE) {
+        __syncthreads();
+        const int jt = j0 + tid;
+        if (jt < N) {
+            s_x[tid] = gx[jt]; s_y[tid] = gy[jt]; s_z[tid] = gz[jt];
+            s_q[tid] = qs[jt]; s_s[tid] = sgs[jt]; s_tc[tid] = tcs[jt]; s_body[tid] = bodies[jt];
+        } else {
+            s_body[tid] = -2;
+        }
+        __syncthreads();
+        if (!vi) continue;
+        const int nj = min(FULL_TILE, N - j0);
+        for (int k = 0; k < nj; ++k) {
+            if (s_body[k] == bi) continue;
+            const int tcj = s_tc[k];
+            const bool cj = (tcj >> 8) != 0;
+            const float dx = min_image(xi - s_x[k], S.box, S.inv_box);
+            const float dy = min_image(yi - s_y[k], S.box, S.inv_box);
+            const float dz = min_image(zi - s_z[k], S.box, S.inv_box);
+            const float r2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-36f);
+            const float rinv = __builtin_amdgcn_rsqf(r2);
+            const float r = r2 * rinv;
+            float dedr;
+            if (ci || cj) {
+                const float t = ff.crow_s2 * (rinv * rinv);
+                const float t6 = t * t * t;
+                dedr = r2 < ff.crow_cut2 ? -6.0f * ff.c_crow * t6 * rinv : 0.0f;
+            } else {
+                const int ti = tci & 0xff, tj = tcj & 0xff;
+                const float traw = s_tab[ti * REMC_NTYPES + tj];
+                const float eij = af.lambda * (traw - af.e0);
+                const float sij = 0.5f * (si + s_s[k]);
+                const float s2 = sij * sij * (rinv * rinv);
+                const float x6 = s2 * s2 * s2;
+                const float base = 4.0f * x6 * (x6 - 1.0f);
+                const float dbase = 4.0f * (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv);
+                const bool flip = (eij > 0.0f) & (x6 > 0.5f);
+                const float dlj = flip ? eij * dbase : -eij * dbase;
+                const float dlj_de = flip ? base + 2.0f : -base;
+                const float dh = af.c_dh * qi * s_q[k] * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+                const float ddh = -dh * (1.0f / af.xi + rinv);
+                dedr = af.c_lj * dlj + ddh;
+                // each unordered pair once for the parameter sums
+                if (s_body[k] > bi) {
+                    const float de_deij = af.c_lj * dlj_de;
+                    dl += (double)(de_deij * (traw - af.e0));
+                    de0 += (double)(-de_deij * af.lambda);
+                    dxi += (double)(dh * r * inv_xi2);
+                }
+            }
+            const float kf = dedr * rinv;
+            gxa += (double)(kf * dx);
+            gya += (double)(kf * dy);
+            gza += (double)(kf * dz);
+        }
+    }
+    if (vi) {
+        bead_grad[3 * i + 0] = gxa;
+        bead_grad[3 * i + 1] = gya;
+        bead_grad[3 * i + 2] = gza;
+    }
+    const double v0 = wave_sum_dpp(dl), v1 = wave_sum_dpp(de0), v2 = wave_sum_dpp(dxi);
+    if ((tid & 63) == 0) {
+        s_red[0][tid >> 6] = v0;
+        s_red[1][tid >> 6] = v1;
+        s_red[2][tid >> 6] = v2;
+    }
+    __syncthreads();
+    if (tid < 3) {
+        double t = 0.0;
+        for (int w = 0; w < FULL_TILE / 64; ++w) t += s_red[tid][w];
+        param_part[3 * blockIdx.x + tid] = t;
+    }
+}
+
+__global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad, const double *param_part, int n_tiles,
+                                 double *force_torque, double *dparams)
+{
+    const DevRep R = p.rep[replica];
+    const DevSys S = p.sys[R.sys];
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m == 0) {
+        for (int k = 0; k < 3; ++k) {
+            double t = 0.0;
+            for (int a = 0; a < n_tiles; ++a) t += param_part[3 * a + k];
+            dparams[k] = t;
+        }
+    }
+    if (m >= S.n_bodies) return;
+    const int b0 = p.bstart[S.body_off + m], b1 = p.bend[S.body_off + m];
+    const float *rx = p.rx + R.bead_off, *ry = p.ry + R.bead_off, *rz = p.rz + R.bead_off;
+    double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
+    for (int i = b0; i < b1; ++i) {
+        const double f0 = -bead_grad[3 * i], f1 = -bead_grad[3 * i + 1], f2 = -bead_grad[3 * i + 2];
+        F[0] += f0; F[1] += f1; F[2] += f2;
+        const double ax = rx[i], ay = ry[i], az = rz[i];
+        T[0] += ay * f2 - az * f1;
+        T[1] += az * f0 - ax * f2;
+        T[2] += ax * f1 - ay * f0;
+    }
+    for (int k = 0; k < 3; ++k) {
+        force_torque[6 * m + k] = F[k];
+        force_torque[6 * m + 3 + k] = T[k];
+    }
+}
+
+hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
+                          double *bead_grad, double *param_part, double *force_torque, double *dparams, hipStream_t s)
+{
+    const int tiles = (n_beads + FULL_TILE - 1) / FULL_TILE;
+    const AdjFF af = *(const AdjFF *)adj_ff;
+    hipLaunchKernelGGL(k_adjoint_beads, dim3(tiles), dim3(FULL_TILE), 0, s, p, af, replica, bead_grad, param_part);
+    hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 63) / 64), dim3(64), 0, s, p, replica, bead_grad, param_part,
+                       tiles, force_torque, dparams);
+    return hipGetLastError();
+}
+
+size_t adjoint_ff_size() { return sizeof(AdjFF); }
+
+void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh)
+{
+    AdjFF *a = (AdjFF *)dst;
+    for (int k = 0; k < REMC_NTYPES * REMC_NTYPES; ++k) a->table[k] = table[k];
+    a->lambda = (float)lambda;
+    a->e0 = (float)e0;
+    a->xi = (float)xi;
+    a->c_lj = (float)c_lj;
+    a->c_dh = (float)c_dh;
+}
+
 } // namespace remc
