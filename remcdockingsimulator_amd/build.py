@@ -14,8 +14,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libremc.so")
+SIM = os.path.join(HERE, "remc_sim")
 
 SOURCES = ["remc_kernels.hip", "remc_api.hip"]
+HOST_SOURCES = ["host/remc_facade.cpp", "host/remc_sim.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -38,6 +40,24 @@ def hipcc_path():
     raise RuntimeError("hipcc not found")
 
 
+# host-side facade and command-line driver: plain C++ over the C ABI
+GXX_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unknown-pragmas"]
+
+
+def build_sim(verbose=False):
+    """remc_sim: the REMC driver executable, linked to libremc.so beside it."""
+    inc = ["-I", os.path.join(REPO, "include"), "-I", CSRC, "-I", os.path.join(CSRC, "host")]
+    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
+    tmp = SIM + ".tmp"
+    cmd = [os.environ.get("CXX", "g++")] + GXX_FLAGS + inc + srcs + ["-L", HERE, "-lremc",
+                                                                    "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, SIM)
+    return SIM
+
+
 def build(verbose=False):
     inc = ["-I", os.path.join(REPO, "include"), "-I", CSRC]
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
@@ -47,6 +67,7 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_sim(verbose)
     return LIB
 
 

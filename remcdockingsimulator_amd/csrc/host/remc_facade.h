@@ -1,0 +1,177 @@
+// remc_facade.h -- C++ drop-in for the reference's host-side API of the REMC
+// path: AminoAcids, Molecule, Replica, the simulation-config file, the REMC
+// driver and its output files.  Compute goes through the C ABI (remc.h).
+//
+// Mirrors (paths relative to "simulation engine/src/"):
+//   AminoAcids           AminoAcids.cpp:25-195
+//   Molecule             Molecule.h:77-95, Molecule.cpp:88-346,430-830
+//   Replica              Replica.h, Replica.cpp:145-530,603-745,1717-1780
+//   argdata/getArgs      definitions.h:203-225, main.cpp:193-381
+//   loadArgsFromFile     main.cpp:383-632
+//   REMCSimulation       main.cpp:1106-1567, initSamplingFiles main.cpp:634-713
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "remc.h"
+
+namespace remc_facade {
+
+constexpr float CROWDER_IDENTIFIER = -1.0f;   // definitions.h:92
+constexpr float BOUND_ENERGY_VALUE = -1.1844f; // definitions.h:84
+constexpr float MIN_TRANSLATION = 0.5f, MAX_TRANSLATION = 5.0f; // definitions.h:79-80
+constexpr float MIN_ROTATION = 0.1f, MAX_ROTATION = 0.5f;       // definitions.h:82-83
+
+struct Vector3f {
+    float x = 0, y = 0, z = 0;
+    Vector3f() = default;
+    Vector3f(float a, float b, float c) : x(a), y(b), z(c) {}
+};
+
+struct Vector3double {
+    double x = 0, y = 0, z = 0;
+    Vector3double() = default;
+    Vector3double(double a, double b, double c) : x(a), y(b), z(c) {}
+    void normalizeInPlace(); // vector3f.cpp:36-43
+};
+
+struct Quaternion {
+    double w = 1, x = 0, y = 0, z = 0;
+    Quaternion() = default;
+    Quaternion(double w_, double x_, double y_, double z_) : w(w_), x(x_), y(y_), z(z_) {}
+    Quaternion normalize();                      // Quaternion.cpp:19-27
+    Quaternion operator*(const Quaternion &q) const; // Quaternion.cpp:35-53
+    Vector3f rotateVector(Vector3f v) const;     // Quaternion.cpp:84-101
+};
+
+struct AminoAcid {
+    std::string name, sname;
+    char symbol = ' ';
+    float vanderWaalRadius = 0.f;
+    float electrostaticCharge = 0.f;
+};
+
+class AminoAcids {
+  public:
+    AminoAcid data[REMC_NTYPES];
+    float LJpotentials[REMC_NTYPES][REMC_NTYPES] = {};
+    int count = 0;
+    // The reference exits on a missing file; these return false instead.
+    bool loadAminoAcidData(const char *filename);
+    bool loadLJPotentialData(const char *filename);
+    // -1 when unknown (the reference returns -1 too)
+    int getAminoAcidIndex(const char *name) const;
+    const AminoAcid &get(int i) const { return data[i]; }
+};
+
+struct Residue {
+    int aminoAcidIndex = 0;
+    float electrostaticCharge = 0.f;
+    float vanderWaalRadius = 0.f;
+    Vector3f position, relativePosition;
+    char chainId = ' ';
+    int resSeq = 0;
+};
+
+class Molecule {
+  public:
+    std::vector<Residue> Residues;
+    Vector3f center, position;
+    Quaternion rotation;
+    float moleculeRoleIdentifier = 0.f;
+    std::string filename;
+    int index = 0;
+
+    bool initFromPDB(const char *pdbfilename, const AminoAcids &aas);
+    bool translate(Vector3f v);
+    void setPosition(Vector3f v);
+    bool rotateQ(Vector3double axis, double angle);
+    void setRotation(Quaternion q);
+    void setMoleculeRoleIdentifier(float role);
+    bool isCrowder() const { return moleculeRoleIdentifier == CROWDER_IDENTIFIER; }
+    size_t residueCount() const { return Residues.size(); }
+};
+
+// parameters of a run (definitions.h:203-225)
+struct argdata {
+    int threads = 1, streams = 1, gpus = 1;
+    int MCsteps = 5000, REsteps = 10, replicas = 20;
+    float bound = 150.0f;
+    int sampleFrequency = 5000, sampleStartsAfter = 5000;
+    std::string prependageString = "output";
+    std::string file;
+    bool inputFile = false;
+    int nonCrowders = 0;
+    float temperatureMin = 250.0f, temperatureMax = 500.0f;
+    int pid = 0;
+    uint64_t seed = 0;
+    bool seedGiven = false;
+    std::string logfile;
+    std::string outputDir = "output";
+    std::string dataDir;
+    int device = 0;
+    int blockDim = 0; // MC workgroup size; 0 = autotuned
+    bool dryRun = false;
+    bool help = false;
+    std::string savePdb, finalPdb; // initial / final (lowest temperature) conformation
+};
+
+class Replica {
+  public:
+    std::vector<Molecule> molecules;
+    AminoAcids aminoAcids;
+    float temperature = 300.0f;
+    int label = -1;
+    double potential = 0.0;
+    float boundingValue = 150.0f;
+    float translateStep = 0.5f;
+    double rotateStep = 0.2;
+    int nonCrowderCount = 0;
+    // statistics (Replica.h:119-131)
+    long accept = 0, acceptA = 0, reject = 0, totalAccept = 0, totalAcceptReject = 0;
+    long boundSamples = 0, samplesSinceLastExchange = 0, totalBoundSamples = 0, totalSamples = 0;
+
+    void setAminoAcidData(const AminoAcids &a) { aminoAcids = a; }
+    int loadMolecule(const char *pdbfilename);
+    int loadMolecule(const char *pdbfilename, Vector3f position, Vector3double rotationAxis, double rotationAmount);
+    void setBoundingValue(float v) { boundingValue = v; }
+    void setTemperature(float t) { temperature = t; }
+    void setTranslateStep(float t) { translateStep = t; }
+    void setRotateStep(double r) { rotateStep = r; }
+    void setLabel(int l) { label = l; }
+    int residueCount() const;
+    long countpairs() const;
+    void exchangeReplicas(Replica &r); // Replica.cpp:145-159
+    // Full energy on the GPU (Replica::E / EonDevice); a single-replica context.
+    double E(int device = 0);
+    void saveAsSinglePDB(const char *filename) const; // Replica.cpp:495-530
+
+    // system packing for the C ABI
+    struct Packed {
+        std::vector<int> body_start, type;
+        std::vector<float> charge, sigma, x, y, z, rx, ry, rz, cx, cy, cz;
+        std::vector<unsigned char> crowder;
+        std::vector<double> quat;
+        remc_system_desc desc();
+    };
+    Packed pack() const;
+};
+
+// force field of an AminoAcids table with the reference constants (definitions.h:18-35)
+remc_forcefield makeForceField(const AminoAcids &aas);
+
+// CLI (main.cpp:193-381, the options of this path) and config file
+// (main.cpp:383-632).  Return false on a fatal error (message printed).
+bool getArgs(argdata *d, int argc, char **argv);
+bool loadArgsFromFile(argdata *d, Replica *initial);
+
+// The REMC driver with the reference's sampling and output files
+// (output/<prefix>_<pid>_{fractionBound,boundconformations,acceptance_ratios,
+// exchange_freq}); returns 0 on success.  final_replicas (may be null)
+// receives every replica's end state and statistics.
+int REMCSimulation(Replica *initial, argdata *params, std::vector<Replica> *final_replicas = nullptr);
+
+} // namespace remc_facade

@@ -1,0 +1,91 @@
+"""remc_sim's host side without a device (-q: load and report, no
+simulation): the configuration parser, the Calpha loader, the placement
+transforms and the single-PDB writer against the Python model of the same
+configuration, and the fileindex output ("simulation engine/src/main.cpp:
+383-632", Replica.cpp:495-530)."""
+import os
+
+import pytest
+
+from facade_helpers import PLACEMENTS, python_molecules, run_sim, split_chains, write_config
+from remcdockingsimulator_amd import model
+
+PARAMS = {"mcsteps": 2000, "resteps": 4, "replicas": 4, "samplefrequency": 100, "sampleafter": 0,
+          "boundry": 120, "temperaturemax": 420, "temperaturemin": 290, "prefix": "t1"}
+
+
+@pytest.fixture()
+def configured(tmp_path, aas):
+    paths = split_chains(1, str(tmp_path))
+    cfg = os.path.join(str(tmp_path), "cfg")
+    write_config(cfg, PARAMS)
+    return tmp_path, paths, cfg
+
+
+def test_dry_run_matches_python_model(configured, aas):
+    tmp, paths, cfg = configured
+    out = os.path.join(str(tmp), "out")
+    r = run_sim(["-f", cfg, "-q", "--save-pdb", "init.pdb", "--outdir", out, "--pid", "7"], cwd=str(tmp))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Loaded: 148 residues in 4 molecules" in r.stdout
+    # inter-molecule pairs: (148^2 - (24^2*3 + 76^2)) / 2
+    assert "%d residue interaction pairs." % ((148 ** 2 - (3 * 24 ** 2 + 76 ** 2)) // 2) in r.stdout
+    assert "non-crowder molecules 2" in r.stdout
+    assert "replicas 4" in r.stdout and "bounding box size 120" in r.stdout
+
+    mols = python_molecules(paths, aas)
+    expect = os.path.join(str(tmp), "expect.pdb")
+    model.write_single_pdb(expect, mols, aas, potential=0.0, temperature=300.0)
+    got = open(os.path.join(str(tmp), "init.pdb")).read().splitlines()
+    exp = open(expect).read().splitlines()
+    assert got[0] == "REMARK init.pdb "
+    assert got[1:] == exp[1:]
+
+    idx = open(os.path.join(out, "t1_repulsive_7_fileindex")).read()
+    assert idx == ("index molecule_file_path crowder(Y/N)\n"
+                   " 0 seg_A.pdb N\n 1 seg_B.pdb N\n 2 seg_A.pdb Y\n 3 seg_A.pdb Y\n")
+
+
+def test_negative_angle_skips_rotation_and_tabs_parse(configured, aas):
+    """The tab-separated last line parses; its negative angle leaves the
+    crowder unrotated (quaternion stays the identity)."""
+    tmp, paths, cfg = configured
+    r = run_sim(["-f", cfg, "-q", "--save-pdb", "init.pdb", "--outdir", "o", "--pid", "1"], cwd=str(tmp))
+    assert r.returncode == 0, r.stdout + r.stderr
+    txt = open(os.path.join(str(tmp), "init.pdb")).read()
+    assert "REMARK Rotation relative to input Q(w,x,y,z): 1.000000 0.000000 0.000000 0.000000\n" \
+           "REMARK Centriod position P(x,y,z): 10.000000 100.000000 20.000000" in txt
+
+
+def test_command_line_overrides_and_errors(configured):
+    tmp, paths, cfg = configured
+    # a malformed molecule line is reported and skipped, as the reference does
+    with open(cfg, "a") as f:
+        f.write("q(1,2,3) r(1,0,0,0) seg_A.pdb\n")
+    r = run_sim(["-f", cfg, "-q", "--outdir", "o", "--pid", "2"], cwd=str(tmp))
+    assert r.returncode == 0
+    assert "Failed to parse line" in r.stdout
+    assert "Loaded: 148 residues in 4 molecules" in r.stdout
+    # a missing structure file is an error
+    with open(cfg, "a") as f:
+        f.write("p(1,2,3) r(1,0,0,0) missing.pdb\n")
+    r = run_sim(["-f", cfg, "-q", "--outdir", "o", "--pid", "3"], cwd=str(tmp))
+    assert r.returncode != 0
+    # options without a value and unknown options are rejected
+    assert run_sim(["-f", cfg, "-m"], cwd=str(tmp)).returncode != 0
+    assert run_sim(["--bogus", "1"], cwd=str(tmp)).returncode != 0
+    assert run_sim(["-h"], cwd=str(tmp)).returncode == 0
+    assert run_sim([], cwd=str(tmp)).returncode != 0
+
+
+def test_temperature_swap_and_thread_clamp(tmp_path, aas):
+    """tmax < tmin is swapped and threads are clamped to the replica count
+    (main.cpp:572-583)."""
+    paths = split_chains(2, str(tmp_path))
+    cfg = os.path.join(str(tmp_path), "cfg")
+    write_config(cfg, dict(PARAMS, temperaturemax=280, temperaturemin=400, threads=16), PLACEMENTS[:2])
+    r = run_sim(["-f", cfg, "-q", "--outdir", "o", "--pid", "4"], cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "swapping" in r.stdout
+    assert "maximum temperature 400" in r.stdout and "minimum temperature 280" in r.stdout
+    assert "Too many threads, setting equal to 4" in r.stdout
