@@ -174,6 +174,20 @@ REMC_API int remc_set_state(remc_ctx *ctx, int replica, const float *x, const fl
                    const float *rx, const float *ry, const float *rz,
                    const float *cx, const float *cy, const float *cz, const double *quat);
 
+/* ---------------------------------------------------------- checkpoint --- */
+/* What a checkpoint holds beyond remc_get_state/remc_set_state: the tracked
+ * energies, the accept counters with each replica's RNG step counter, every
+ * system's ladder and exchange state (round counter, offset, swap total).
+ * A fresh context given the saved conformations and these continues the saved
+ * run bit for bit.  Together they replace saveSimulation and the resume path
+ * (main.cpp:89-122, 336-372), whose load never restores a run. */
+REMC_API int remc_set_energies(remc_ctx *ctx, const double *energies);
+REMC_API int remc_set_counters(remc_ctx *ctx, const remc_counters *in);
+/* slot_replica[k] = replica holding rung k; temperatures follow the rungs. */
+REMC_API int remc_set_ladder(remc_ctx *ctx, int sys_id, const int *slot_replica);
+REMC_API int remc_get_exchange_state(remc_ctx *ctx, int sys_id, uint64_t *round, int *offset, uint64_t *swaps);
+REMC_API int remc_set_exchange_state(remc_ctx *ctx, int sys_id, uint64_t round, int offset, uint64_t swaps);
+
 /* ------------------------------------------------------------- adjoint --- */
 /* Per-body force and torque about the body centre (6 doubles per body:
  * Fx Fy Fz Tx Ty Tz, kcal/mol/A and kcal/mol/rad, F = -dE/dx) and the
@@ -204,6 +218,14 @@ REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
  * LDS cache; split_min_pairs (> 0) is the pair-term count from which a move is
  * split across cooperating workgroups (default 8192; 0 keeps the current). */
 REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
+/* Speculative windows of the MC kernel (default on): up to one step per wave
+ * of consecutive moves of small bodies (at most 16 beads, not split across
+ * workgroups) is evaluated at once against the window's starting
+ * conformation and decided in order with exact pair corrections between the
+ * window's moved bodies; used when the replica's beads and bodies fit the
+ * LDS caches.  Decisions and states equal those of one step at a time up to
+ * FP32 rounding.  0 turns the windows off.  No counterpart in the reference. */
+REMC_API int remc_set_mc_window(remc_ctx *ctx, int enable);
 
 #ifdef __cplusplus
 }

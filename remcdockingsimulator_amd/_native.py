@@ -74,6 +74,12 @@ SIGNATURES = {
     "remc_get_ladder": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
     "remc_get_state": (ctypes.c_int, [_vp, ctypes.c_int] + [_fp] * 9 + [_dp]),
     "remc_set_state": (ctypes.c_int, [_vp, ctypes.c_int] + [_fp] * 9 + [_dp]),
+    "remc_set_energies": (ctypes.c_int, [_vp, _dp]),
+    "remc_set_counters": (ctypes.c_int, [_vp, ctypes.POINTER(remc_counters)]),
+    "remc_set_ladder": (ctypes.c_int, [_vp, ctypes.c_int, _ip]),
+    "remc_get_exchange_state": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
+                                               ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]),
+    "remc_set_exchange_state": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]),
     "remc_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "remc_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
@@ -83,6 +89,7 @@ SIGNATURES = {
     "remc_set_mc_coop": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_get_mc_coop": (ctypes.c_int, [_vp, _ip]),
     "remc_set_mc_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "remc_set_mc_window": (ctypes.c_int, [_vp, ctypes.c_int]),
 }
 
 _lib = None

@@ -555,6 +555,10 @@ static void printHelp()
     std::printf("\t--data d:        directory holding AminoAcids and pair_potentials_MJ1996\n");
     std::printf("\t--save-pdb f:    write the initial conformation as a single PDB file\n");
     std::printf("\t--final-pdb f:   write the lowest-temperature replica's final conformation\n");
+    std::printf("\t--coop x:        MC workgroups per replica (1, 2, 4 or 8; default autotuned)\n");
+    std::printf("\t--checkpoint x:  write a checkpoint every x MC steps (rounded to exchange rounds)\n");
+    std::printf("\t--checkpoint-dir d: directory of the checkpoint files (default: checkpoints)\n");
+    std::printf("\t-z|--resume f:   continue the run saved in checkpoint file f\n");
 }
 
 bool getArgs(argdata *d, int argc, char **argv)
@@ -613,9 +617,12 @@ bool getArgs(argdata *d, int argc, char **argv)
         else if (is("--data", nullptr)) d->dataDir = v;
         else if (is("--save-pdb", nullptr)) d->savePdb = v;
         else if (is("--final-pdb", nullptr)) d->finalPdb = v;
+        else if (is("--coop", nullptr)) d->coop = std::atoi(v);
+        else if (is("--checkpoint", nullptr)) d->checkpointFrequency = std::atoi(v);
+        else if (is("--checkpoint-dir", nullptr)) d->checkpointDir = v;
         else if (is("-z", "--resume")) {
-            std::printf("!!! resume from a checkpoint is not supported by this driver\n");
-            return false;
+            d->resume = true;
+            d->resumeFile = v;
         } else {
             std::printf(" * Bad arguments: %s\n", a);
             printHelp();
@@ -767,6 +774,248 @@ bool loadArgsFromFile(argdata *d, Replica *initial)
     return true;
 }
 
+// -------------------------------------------------------- checkpoint ----
+
+bool saveSimulation(const std::string &path, const argdata &p, const Checkpoint &ck)
+{
+    // main.cpp:89-117 layout first, then the exact section
+    const std::string tmp = path + ".tmp";
+    FILE *o = std::fopen(tmp.c_str(), "w");
+    if (!o) {
+        std::printf("Cannot write checkpoint %s\n", tmp.c_str());
+        return false;
+    }
+    const int R = (int)ck.energy.size(), N = ck.nBeads, M = ck.nBodies;
+    std::fprintf(o, "#Saved simulation state file, prependage string, original pid\n");
+    std::fprintf(o, "%s\n", p.file.c_str());
+    std::fprintf(o, "%s_%d\n", p.prependageString.c_str(), p.pid);
+    std::fprintf(o, "%d\n", p.pid);
+    std::fprintf(o, "# threads, streams, gpus, MCsteps, RESteps, sampleFrequency, sampleStartsAfter,nonCrowders, "
+                    "temperatureMin, temperatureMax, currentStep\n");
+    std::fprintf(o, "%d %d %d %d %d %d %d %d %f %f %d\n", p.threads, p.streams, p.gpus, p.MCsteps, p.REsteps,
+                 p.sampleFrequency, p.sampleStartsAfter, p.nonCrowders, (double)p.temperatureMin,
+                 (double)p.temperatureMax, ck.roundsDone * (p.MCsteps / p.REsteps));
+    std::fprintf(o, "#molecules\n%d\n#replicas\n%d\n", M, R);
+    for (int r = 0; r < R; ++r) {
+        std::fprintf(o, "#replica %d, molecules: x,y,z translation. w,x,y,z rotation quaternion\n", r);
+        for (int m = 0; m < M; ++m) {
+            const size_t b = (size_t)r * M + m;
+            std::fprintf(o, "%.7f %.7f %.7f %.7f %.7f %.7f %.7f\n", (double)ck.cx[b], (double)ck.cy[b],
+                         (double)ck.cz[b], ck.quat[4 * b], ck.quat[4 * b + 1], ck.quat[4 * b + 2], ck.quat[4 * b + 3]);
+        }
+        const SlotStats *st = nullptr;
+        for (int k = 0; k < R; ++k)
+            if (ck.ladder[k] == r) st = &ck.slots[k];
+        std::fprintf(o, "#replica %d, stats: accept, acceptA, reject, totalAcceptReject, totalAccept, boundSamples, "
+                        "samplesSinceLastExchange, totalBoundSamples, totalSamples\n", r);
+        std::fprintf(o, "%ld %ld %ld %ld %ld %ld %ld %ld %ld\n", (long)(ck.counters[r].accept - ck.seen[r].accept),
+                     (long)(ck.counters[r].accept_boltz - ck.seen[r].accept_boltz),
+                     (long)(ck.counters[r].reject - ck.seen[r].reject), st->totalAcceptReject, st->totalAccept,
+                     st->boundSamples, st->samplesSinceLastExchange, st->totalBoundSamples, st->totalSamples);
+    }
+    std::fprintf(o, "#exact: seed rounds_done total_exchanges total_tests replicas box beads bodies\n");
+    std::fprintf(o, "%llu %d %d %d %d %a %d %d\n", (unsigned long long)ck.seed, ck.roundsDone, ck.totalExchanges,
+                 ck.totalTests, R, (double)ck.box, N, M);
+    std::fprintf(o, "#exact: ladder (replica on each rung), exchange round offset swaps\n");
+    for (int k = 0; k < R; ++k) std::fprintf(o, "%d%c", ck.ladder[k], k + 1 < R ? ' ' : '\n');
+    std::fprintf(o, "%llu %d %llu\n", (unsigned long long)ck.exRound, ck.exOffset, (unsigned long long)ck.exSwaps);
+    std::fprintf(o, "#exact: per replica energy, accept accept_boltz reject rng_steps, the same at the last report\n");
+    for (int r = 0; r < R; ++r)
+        std::fprintf(o, "%a %llu %llu %llu %llu %llu %llu %llu\n", ck.energy[r],
+                     (unsigned long long)ck.counters[r].accept, (unsigned long long)ck.counters[r].accept_boltz,
+                     (unsigned long long)ck.counters[r].reject, (unsigned long long)ck.counters[r].steps,
+                     (unsigned long long)ck.seen[r].accept, (unsigned long long)ck.seen[r].accept_boltz,
+                     (unsigned long long)ck.seen[r].reject);
+    std::fprintf(o, "#exact: per rung totalAccept totalAcceptReject boundSamples samplesSinceLastExchange "
+                    "totalBoundSamples totalSamples\n");
+    for (const SlotStats &st : ck.slots)
+        std::fprintf(o, "%ld %ld %ld %ld %ld %ld\n", st.totalAccept, st.totalAcceptReject, st.boundSamples,
+                     st.samplesSinceLastExchange, st.totalBoundSamples, st.totalSamples);
+    for (int r = 0; r < R; ++r) {
+        std::fprintf(o, "#exact replica %d: beads x y z rx ry rz, then bodies cx cy cz qw qx qy qz\n", r);
+        for (int i = 0; i < N; ++i) {
+            const size_t b = (size_t)r * N + i;
+            std::fprintf(o, "%a %a %a %a %a %a\n", (double)ck.x[b], (double)ck.y[b], (double)ck.z[b], (double)ck.rx[b],
+                         (double)ck.ry[b], (double)ck.rz[b]);
+        }
+        for (int m = 0; m < M; ++m) {
+            const size_t b = (size_t)r * M + m;
+            std::fprintf(o, "%a %a %a %a %a %a %a\n", (double)ck.cx[b], (double)ck.cy[b], (double)ck.cz[b],
+                         ck.quat[4 * b], ck.quat[4 * b + 1], ck.quat[4 * b + 2], ck.quat[4 * b + 3]);
+        }
+    }
+    const bool good = std::fflush(o) == 0 && !std::ferror(o);
+    std::fclose(o);
+    if (!good || std::rename(tmp.c_str(), path.c_str()) != 0) {
+        std::printf("Cannot write checkpoint %s\n", path.c_str());
+        return false;
+    }
+    return true;
+}
+
+namespace {
+
+// Reads the non-comment lines of a checkpoint in order.
+struct LineReader {
+    std::vector<std::string> lines;
+    size_t i = 0;
+    const char *next()
+    {
+        while (i < lines.size() && (lines[i].empty() || lines[i][0] == '#')) ++i;
+        return i < lines.size() ? lines[i++].c_str() : nullptr;
+    }
+};
+
+bool parse_doubles(const char *s, double *out, int n)
+{
+    char *end = nullptr;
+    for (int k = 0; k < n; ++k) {
+        out[k] = std::strtod(s, &end);
+        if (end == s) return false;
+        s = end;
+    }
+    return true;
+}
+
+bool parse_u64s(const char *s, unsigned long long *out, int n)
+{
+    char *end = nullptr;
+    for (int k = 0; k < n; ++k) {
+        out[k] = std::strtoull(s, &end, 10);
+        if (end == s) return false;
+        s = end;
+    }
+    return true;
+}
+
+} // namespace
+
+bool loadSimulation(const std::string &path, argdata *p, Checkpoint *ck)
+{
+    LineReader in;
+    if (!read_lines(path.c_str(), in.lines)) {
+        std::printf("Cannot open checkpoint %s\n", path.c_str());
+        return false;
+    }
+    auto bad = [&](const char *what) {
+        std::printf("Checkpoint %s: malformed %s\n", path.c_str(), what);
+        return false;
+    };
+    const char *s;
+    if (!(s = in.next())) return bad("config file line");
+    p->file = s;
+    p->inputFile = true;
+    if (!(s = in.next())) return bad("prefix line");
+    std::string prefix_pid = s;
+    if (!(s = in.next())) return bad("pid line");
+    p->pid = std::atoi(s);
+    const std::string suffix = "_" + std::to_string(p->pid);
+    if (prefix_pid.size() <= suffix.size() || prefix_pid.compare(prefix_pid.size() - suffix.size(), suffix.size(), suffix))
+        return bad("prefix line");
+    p->prependageString = prefix_pid.substr(0, prefix_pid.size() - suffix.size());
+    if (!(s = in.next())) return bad("parameter line");
+    int current = 0;
+    if (std::sscanf(s, "%d %d %d %d %d %d %d %d %f %f %d", &p->threads, &p->streams, &p->gpus, &p->MCsteps, &p->REsteps,
+                    &p->sampleFrequency, &p->sampleStartsAfter, &p->nonCrowders, &p->temperatureMin,
+                    &p->temperatureMax, &current) != 11)
+        return bad("parameter line");
+    if (!(s = in.next())) return bad("molecule count");
+    const int M = std::atoi(s);
+    if (!(s = in.next())) return bad("replica count");
+    const int R = std::atoi(s);
+    if (M < 1 || R < 1) return bad("counts");
+    // the reference-layout block (rounded poses, stats): the exact section below supersedes it
+    for (int r = 0; r < R; ++r)
+        for (int k = 0; k < M + 1; ++k)
+            if (!in.next()) return bad("replica block");
+    if (!(s = in.next())) return bad("exact header");
+    unsigned long long seed = 0;
+    int n_rep = 0, N = 0, Mx = 0;
+    double box = 0;
+    {
+        char boxs[64];
+        if (std::sscanf(s, "%llu %d %d %d %d %63s %d %d", &seed, &ck->roundsDone, &ck->totalExchanges, &ck->totalTests,
+                        &n_rep, boxs, &N, &Mx) != 8)
+            return bad("exact header");
+        box = std::strtod(boxs, nullptr);
+    }
+    if (n_rep != R || Mx != M || N < 1) return bad("exact sizes");
+    ck->seed = seed;
+    ck->box = (float)box;
+    ck->nBeads = N;
+    ck->nBodies = M;
+    p->seed = seed;
+    p->seedGiven = true;
+    p->replicas = R;
+    p->bound = (float)box;
+    if (!(s = in.next())) return bad("ladder");
+    {
+        ck->ladder.assign(R, 0);
+        char *end = nullptr;
+        for (int k = 0; k < R; ++k) {
+            ck->ladder[k] = (int)std::strtol(s, &end, 10);
+            if (end == s) return bad("ladder");
+            s = end;
+        }
+    }
+    if (!(s = in.next())) return bad("exchange state");
+    {
+        unsigned long long rd = 0, sw = 0;
+        if (std::sscanf(s, "%llu %d %llu", &rd, &ck->exOffset, &sw) != 3) return bad("exchange state");
+        ck->exRound = rd;
+        ck->exSwaps = sw;
+    }
+    ck->energy.assign(R, 0.0);
+    ck->counters.assign(R, remc_counters{});
+    ck->seen.assign(R, remc_counters{});
+    for (int r = 0; r < R; ++r) {
+        if (!(s = in.next())) return bad("replica counters");
+        char *end = nullptr;
+        ck->energy[r] = std::strtod(s, &end);
+        if (end == s) return bad("replica energy");
+        unsigned long long v[7];
+        if (!parse_u64s(end, v, 7)) return bad("replica counters");
+        ck->counters[r] = remc_counters{v[0], v[1], v[2], v[3]};
+        ck->seen[r] = remc_counters{v[4], v[5], v[6], 0};
+    }
+    ck->slots.assign(R, SlotStats{});
+    for (int k = 0; k < R; ++k) {
+        if (!(s = in.next())) return bad("rung statistics");
+        SlotStats &st = ck->slots[k];
+        if (std::sscanf(s, "%ld %ld %ld %ld %ld %ld", &st.totalAccept, &st.totalAcceptReject, &st.boundSamples,
+                        &st.samplesSinceLastExchange, &st.totalBoundSamples, &st.totalSamples) != 6)
+            return bad("rung statistics");
+    }
+    const size_t nb = (size_t)R * N, nm = (size_t)R * M;
+    for (auto *v : {&ck->x, &ck->y, &ck->z, &ck->rx, &ck->ry, &ck->rz}) v->assign(nb, 0.f);
+    for (auto *v : {&ck->cx, &ck->cy, &ck->cz}) v->assign(nm, 0.f);
+    ck->quat.assign(4 * nm, 0.0);
+    for (int r = 0; r < R; ++r) {
+        for (int i = 0; i < N; ++i) {
+            double v[6];
+            if (!(s = in.next()) || !parse_doubles(s, v, 6)) return bad("bead line");
+            const size_t b = (size_t)r * N + i;
+            ck->x[b] = (float)v[0];
+            ck->y[b] = (float)v[1];
+            ck->z[b] = (float)v[2];
+            ck->rx[b] = (float)v[3];
+            ck->ry[b] = (float)v[4];
+            ck->rz[b] = (float)v[5];
+        }
+        for (int m = 0; m < M; ++m) {
+            double v[7];
+            if (!(s = in.next()) || !parse_doubles(s, v, 7)) return bad("body line");
+            const size_t b = (size_t)r * M + m;
+            ck->cx[b] = (float)v[0];
+            ck->cy[b] = (float)v[1];
+            ck->cz[b] = (float)v[2];
+            for (int t = 0; t < 4; ++t) ck->quat[4 * b + t] = v[3 + t];
+        }
+    }
+    p->resume = true;
+    return true;
+}
+
 // ------------------------------------------------------- REMC driver ----
 
 namespace {
@@ -788,26 +1037,29 @@ FILE *open_append(const std::string &path, const char *mode)
     return f;
 }
 
-bool initSamplingFiles(const argdata *d, SamplingFiles &s)
+bool initSamplingFiles(const argdata *d, SamplingFiles &s, bool headers)
 {
-    // main.cpp:634-692: file names and header lines
+    // main.cpp:634-692: file names and header lines (a resumed run appends
+    // to the files of the run it continues, without new headers)
+    mkdir(d->outputDir.c_str(), 0755);
     if (!(s.fractionBound = open_append(out_path(d, "fractionBound"), "at"))) return false;
-    std::fprintf(s.fractionBound, "# Fraction Bound\n#Iteration InstantaniousAve CumulativeAve\n");
+    if (headers) std::fprintf(s.fractionBound, "# Fraction Bound\n#Iteration InstantaniousAve CumulativeAve\n");
     if (!(s.boundConformations = open_append(out_path(d, "boundconformations"), "a+"))) return false;
-    std::fprintf(s.boundConformations,
-                 "# iteration; complex free energy (replica free energy); temperature;\n# molecule: rotation(w,x,y,z) "
-                 "position(x,y,z) 1 line per other molecule in the bound state\n");
+    if (headers)
+        std::fprintf(s.boundConformations,
+                     "# iteration; complex free energy (replica free energy); temperature;\n# molecule: rotation(w,x,y,z) "
+                     "position(x,y,z) 1 line per other molecule in the bound state\n");
     if (!(s.acceptanceRatio = open_append(out_path(d, "acceptance_ratios"), "a+"))) return false;
-    std::fprintf(s.acceptanceRatio, "# Iteration AcceptanceRatio\n");
+    if (headers) std::fprintf(s.acceptanceRatio, "# Iteration AcceptanceRatio\n");
     if (!(s.exchangeFrequency = open_append(out_path(d, "exchange_freq"), "a+"))) return false;
-    std::fprintf(s.exchangeFrequency, "# Iteration continous_ratio, inst_ratio\n");
+    if (headers) std::fprintf(s.exchangeFrequency, "# Iteration continous_ratio, inst_ratio\n");
     std::fflush(nullptr);
     return true;
 }
 
 } // namespace
 
-int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_replicas)
+int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_replicas, const Checkpoint *resume)
 {
     // main.cpp:1106-1567 on one device context: the replicas' MC rounds run
     // as batched launches, exchange rounds on the device; sampling, the
@@ -863,30 +1115,59 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
     if (!ok(remc_system_add(ctx.c, &desc, &sys), "remc_system_add")) return 1;
     if (!ok(remc_replicas_add(ctx.c, sys, R, T.data(), ts.data(), rs.data(), &first), "remc_replicas_add")) return 1;
     if (p->blockDim && !ok(remc_set_mc_block(ctx.c, p->blockDim), "remc_set_mc_block")) return 1;
-
-    SamplingFiles files;
-    if (!initSamplingFiles(p, files)) return 1;
+    if (p->coop && !ok(remc_set_mc_coop(ctx.c, p->coop), "remc_set_mc_coop")) return 1;
+    const int M = (int)initial->molecules.size(), N = (int)packed.x.size();
 
     // statistics that follow a temperature live with its ladder slot; the
     // accept/reject interval counters stay with the replica (Replica.cpp:145-159)
-    struct SlotStats {
-        long totalAccept = 0, totalAcceptReject = 0, boundSamples = 0, samplesSinceLastExchange = 0,
-             totalBoundSamples = 0, totalSamples = 0;
-    };
     std::vector<SlotStats> slot(R);
     std::vector<remc_counters> seen(R), now(R);
     std::memset(seen.data(), 0, sizeof(remc_counters) * R);
     std::vector<int> ladder(R);
     std::vector<double> energies(R), temps(R), noncrowder(R);
+    int totalExchanges = 0, totalTests = 0, startRound = 1;
 
-    std::fprintf(files.fractionBound, "Iteration:  ");
-    std::fprintf(files.acceptanceRatio, "Iteration:  ");
-    for (int k = 0; k < R; ++k) {
-        std::fprintf(files.fractionBound, "%0.1fKi %0.1fKc ", T[k], T[k]);
-        std::fprintf(files.acceptanceRatio, "%0.1fKi %0.1fKc ", T[k], T[k]);
+    if (resume) {
+        // the saved run's state, replica by replica, and its bookkeeping
+        const Checkpoint &ck = *resume;
+        if ((int)ck.energy.size() != R || ck.nBeads != N || ck.nBodies != M || (int)ck.ladder.size() != R) {
+            std::printf("! checkpoint sizes (%d replicas, %d beads, %d bodies) do not match the configuration\n",
+                        (int)ck.energy.size(), ck.nBeads, ck.nBodies);
+            return 1;
+        }
+        for (int r = 0; r < R; ++r) {
+            const size_t b = (size_t)r * N, m = (size_t)r * M;
+            if (!ok(remc_set_state(ctx.c, r, &ck.x[b], &ck.y[b], &ck.z[b], &ck.rx[b], &ck.ry[b], &ck.rz[b], &ck.cx[m],
+                                   &ck.cy[m], &ck.cz[m], &ck.quat[4 * m]),
+                    "remc_set_state"))
+                return 1;
+        }
+        if (!ok(remc_set_energies(ctx.c, ck.energy.data()), "remc_set_energies")) return 1;
+        if (!ok(remc_set_counters(ctx.c, ck.counters.data()), "remc_set_counters")) return 1;
+        if (!ok(remc_set_ladder(ctx.c, sys, ck.ladder.data()), "remc_set_ladder")) return 1;
+        if (!ok(remc_set_exchange_state(ctx.c, sys, ck.exRound, ck.exOffset, ck.exSwaps), "remc_set_exchange_state"))
+            return 1;
+        slot = ck.slots;
+        seen = ck.seen;
+        totalExchanges = ck.totalExchanges;
+        totalTests = ck.totalTests;
+        startRound = ck.roundsDone + 1;
+        std::printf("--- Resuming after exchange round %d (MC step %d) ---\n", ck.roundsDone,
+                    ck.roundsDone * (p->MCsteps / p->REsteps));
     }
-    std::fprintf(files.fractionBound, "\t\n");
-    std::fprintf(files.acceptanceRatio, "\n");
+
+    SamplingFiles files;
+    if (!initSamplingFiles(p, files, !resume)) return 1;
+    if (!resume) {
+        std::fprintf(files.fractionBound, "Iteration:  ");
+        std::fprintf(files.acceptanceRatio, "Iteration:  ");
+        for (int k = 0; k < R; ++k) {
+            std::fprintf(files.fractionBound, "%0.1fKi %0.1fKc ", T[k], T[k]);
+            std::fprintf(files.acceptanceRatio, "%0.1fKi %0.1fKc ", T[k], T[k]);
+        }
+        std::fprintf(files.fractionBound, "\t\n");
+        std::fprintf(files.acceptanceRatio, "\n");
+    }
 
     int mcstepsPerRE = p->MCsteps / p->REsteps;
     if (mcstepsPerRE < 1) {
@@ -907,7 +1188,6 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
         }
     }
     const bool has_crowders = initial->nonCrowderCount != (int)initial->molecules.size();
-    const int M = (int)initial->molecules.size(), N = (int)packed.x.size();
     std::vector<float> cx(M), cy(M), cz(M);
     std::vector<double> quat(4 * M);
 
@@ -946,11 +1226,52 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
         return true;
     };
 
+    // checkpoints fall on exchange rounds (main.cpp:1313-1325)
+    int cpf = p->checkpointFrequency;
+    if (cpf > 0 && cpf % mcstepsPerRE != 0)
+        cpf = cpf < mcstepsPerRE ? mcstepsPerRE : (int)std::ceil(float(cpf) / float(mcstepsPerRE)) * mcstepsPerRE;
+    const std::string cpath = p->checkpointDir + "/" + p->prependageString + "_" + std::to_string(p->pid);
+    if (cpf > 0) {
+        mkdir(p->checkpointDir.c_str(), 0755);
+        std::printf("--- The simulation will checkpoint every %d monte carlo steps ---.\n", cpf);
+    }
+    auto checkpoint = [&](int roundsDone) -> bool {
+        Checkpoint ck;
+        ck.roundsDone = roundsDone;
+        ck.totalExchanges = totalExchanges;
+        ck.totalTests = totalTests;
+        ck.seed = p->seed;
+        ck.box = p->bound;
+        ck.nBeads = N;
+        ck.nBodies = M;
+        ck.energy.assign(R, 0.0);
+        ck.counters.assign(R, remc_counters{});
+        ck.ladder.assign(R, 0);
+        if (!ok(remc_get_energies(ctx.c, ck.energy.data()), "remc_get_energies") ||
+            !ok(remc_get_counters(ctx.c, ck.counters.data()), "remc_get_counters") ||
+            !ok(remc_get_ladder(ctx.c, sys, ck.ladder.data()), "remc_get_ladder") ||
+            !ok(remc_get_exchange_state(ctx.c, sys, &ck.exRound, &ck.exOffset, &ck.exSwaps), "remc_get_exchange_state"))
+            return false;
+        ck.seen = seen;
+        ck.slots = slot;
+        const size_t nb = (size_t)R * N, nm = (size_t)R * M;
+        for (auto *v : {&ck.x, &ck.y, &ck.z, &ck.rx, &ck.ry, &ck.rz}) v->assign(nb, 0.f);
+        for (auto *v : {&ck.cx, &ck.cy, &ck.cz}) v->assign(nm, 0.f);
+        ck.quat.assign(4 * nm, 0.0);
+        for (int r = 0; r < R; ++r) {
+            const size_t b = (size_t)r * N, m = (size_t)r * M;
+            if (!ok(remc_get_state(ctx.c, r, &ck.x[b], &ck.y[b], &ck.z[b], &ck.rx[b], &ck.ry[b], &ck.rz[b], &ck.cx[m],
+                                   &ck.cy[m], &ck.cz[m], &ck.quat[4 * m]),
+                    "remc_get_state"))
+                return false;
+        }
+        return saveSimulation(cpath, *p, ck);
+    };
+
     std::printf("--- Staring simulation ---.\n");
     const int rounds = (p->MCsteps + mcstepsPerRE - 1) / mcstepsPerRE;
-    int totalExchanges = 0, totalTests = 0;
-    int mcstep = 0;
-    for (int round = 1; round <= rounds; ++round) {
+    int mcstep = (startRound - 1) * mcstepsPerRE;
+    for (int round = startRound; round <= rounds; ++round) {
         for (int s = 0; s < mcstepsPerRE / p->sampleFrequency; ++s) {
             if (!ok(remc_mc_steps(ctx.c, p->sampleFrequency), "remc_mc_steps")) return 1;
             if (mcstep + s * p->sampleFrequency >= p->sampleStartsAfter && !sample_all(mcstep + s * p->sampleFrequency))
@@ -1001,6 +1322,11 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
         std::fprintf(files.exchangeFrequency, "%10d %6.4f %6.4f\n", round, (double)(float(totalExchanges) / float(totalTests)),
                      (double)(float(swaps) / float(tests)));
         std::fflush(files.exchangeFrequency);
+        if (cpf > 0 && (round * mcstepsPerRE) % cpf == 0) {
+            std::fflush(nullptr);
+            if (!checkpoint(round)) return 1;
+            std::printf("--- Checkpoint after exchange round %d: %s ---\n", round, cpath.c_str());
+        }
     }
     std::printf("--- Replica Exchanges Complete.---\n--- All threads complete.---\n");
 

@@ -117,7 +117,44 @@ struct argdata {
     bool dryRun = false;
     bool help = false;
     std::string savePdb, finalPdb; // initial / final (lowest temperature) conformation
+    int coop = 0;                  // MC workgroups per replica; 0 = autotuned
+    int checkpointFrequency = 0;   // MC steps between checkpoints; 0 = none
+    std::string checkpointDir = "checkpoints";
+    bool resume = false;
+    std::string resumeFile;
 };
+
+// Statistics that follow a temperature rung: they swap with the temperature
+// on an exchange (Replica.h:119-131, Replica.cpp:145-159).
+struct SlotStats {
+    long totalAccept = 0, totalAcceptReject = 0, boundSamples = 0, samplesSinceLastExchange = 0,
+         totalBoundSamples = 0, totalSamples = 0;
+};
+
+// Everything a run needs to continue bit for bit: the reference's checkpoint
+// content (saveSimulation, main.cpp:89-117) plus the exact state (bead
+// coordinates, tracked energies, RNG step counters, ladder, exchange state).
+struct Checkpoint {
+    int roundsDone = 0, totalExchanges = 0, totalTests = 0;
+    uint64_t seed = 0;
+    float box = 0.f;
+    std::vector<int> ladder; // slot -> replica
+    uint64_t exRound = 0, exSwaps = 0;
+    int exOffset = 0;
+    std::vector<double> energy;
+    std::vector<remc_counters> counters, seen; // device counters; values at the last report
+    std::vector<SlotStats> slots;
+    int nBeads = 0, nBodies = 0;
+    std::vector<float> x, y, z, rx, ry, rz; // replica-major, nBeads each
+    std::vector<float> cx, cy, cz;          // replica-major, nBodies each
+    std::vector<double> quat;               // replica-major, 4 nBodies each
+};
+
+// Checkpoint file: the reference's layout (main.cpp:89-117) followed by the
+// exact section, floating-point values in hexadecimal so they read back
+// unchanged.  loadSimulation fills the run parameters and the state.
+bool saveSimulation(const std::string &path, const argdata &p, const Checkpoint &ck);
+bool loadSimulation(const std::string &path, argdata *p, Checkpoint *ck);
 
 class Replica {
   public:
@@ -171,7 +208,9 @@ bool loadArgsFromFile(argdata *d, Replica *initial);
 // The REMC driver with the reference's sampling and output files
 // (output/<prefix>_<pid>_{fractionBound,boundconformations,acceptance_ratios,
 // exchange_freq}); returns 0 on success.  final_replicas (may be null)
-// receives every replica's end state and statistics.
-int REMCSimulation(Replica *initial, argdata *params, std::vector<Replica> *final_replicas = nullptr);
+// receives every replica's end state and statistics; resume (may be null)
+// continues a run from its checkpoint.
+int REMCSimulation(Replica *initial, argdata *params, std::vector<Replica> *final_replicas = nullptr,
+                   const Checkpoint *resume = nullptr);
 
 } // namespace remc_facade

@@ -34,6 +34,13 @@ int main(int argc, char **argv)
     p.pid = (int)getpid();
     if (!getArgs(&p, argc, argv)) return 1;
     if (p.help) return 0;
+    Checkpoint ck;
+    if (p.resume) {
+        // the checkpoint names the configuration and holds the run's parameters
+        if (!loadSimulation(p.resumeFile, &p, &ck)) return 1;
+        std::printf("The simulation will resume from the checkpoint file: %s\nOriginal input: %s\n",
+                    p.resumeFile.c_str(), p.file.c_str());
+    }
     if (!p.inputFile) {
         std::printf("! no configuration file given (-f); see -h\n");
         return 1;
@@ -48,9 +55,11 @@ int main(int argc, char **argv)
     Replica initial;
     initial.setAminoAcidData(aas);
     initial.setBoundingValue(150.0f);
+    p.nonCrowders = 0;
     if (!loadArgsFromFile(&p, &initial)) return 1;
-    // second pass: command-line values override the file (main.cpp:1686)
     const int nonCrowders = p.nonCrowders;
+    if (p.resume && !loadSimulation(p.resumeFile, &p, &ck)) return 1; // its parameters win over the file's
+    // second pass: command-line values override the file (main.cpp:1686)
     if (!getArgs(&p, argc, argv)) return 1;
     p.nonCrowders = nonCrowders;
     initial.setBoundingValue(p.bound);
@@ -75,7 +84,7 @@ int main(int argc, char **argv)
     std::printf("RNG seed %llu\n", (unsigned long long)p.seed);
     std::printf("Output files will be prefixed by %s_%d\n", p.prependageString.c_str(), p.pid);
     std::vector<Replica> finals;
-    int rc = REMCSimulation(&initial, &p, p.finalPdb.empty() ? nullptr : &finals);
+    int rc = REMCSimulation(&initial, &p, p.finalPdb.empty() ? nullptr : &finals, p.resume ? &ck : nullptr);
     if (rc != 0) return rc;
     if (!p.finalPdb.empty()) {
         size_t lo = 0;

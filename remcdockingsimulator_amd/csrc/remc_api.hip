@@ -124,6 +124,7 @@ struct remc_ctx {
     int n_cu = 0;
     int lds_per_block = 0, lds_per_cu = 0;
     bool lds_cache = true;   // partner-bead LDS cache of the MC kernel allowed
+    bool mc_window = true;   // speculative windows of small moves allowed
     long long min_move_pairs = -1;
     int max_bodies = 0, max_body_beads = 0;
 
@@ -494,7 +495,7 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
     const int R = (int)c->reps.size();
     const long long wgs = (long long)R * coop;
     const int per_cu = (int)std::max<long long>(1, (wgs + c->n_cu - 1) / c->n_cu);
-    const int static_lds = 16 * 1024; // etab, draws, reduction scratch and slack
+    const int static_lds = 24 * 1024; // etab, draws, window staging, reduction scratch and slack
     long long budget = std::min<long long>(c->lds_per_block, c->lds_per_cu / per_cu) - static_lds;
     p.lds_mb = std::max(1, c->max_body_beads);
     budget -= 36LL * p.lds_mb;
@@ -502,6 +503,9 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
     p.lds_nbody = 0;
     p.cache_slice = 0;
     p.cache_bodies = 0;
+    p.cache_rel = 0;
+    p.cache_quat = 0;
+    p.spec = 0;
     if (!c->lds_cache || budget <= 0) return;
     const long long full = 24LL * c->max_beads;
     const bool all_split = coop > 1 && c->min_move_pairs >= c->coop_min_pairs;
@@ -519,7 +523,20 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
     if (20LL * c->max_bodies <= budget) {
         p.lds_nbody = c->max_bodies;
         p.cache_bodies = 1;
+        budget -= 20LL * c->max_bodies;
+    } else {
+        return;
     }
+    if (!p.cache_slice && 12LL * c->max_beads <= budget) {
+        p.cache_rel = 1;
+        budget -= 12LL * c->max_beads;
+    }
+    if (32LL * c->max_bodies <= budget) {
+        p.cache_quat = 1;
+        budget -= 32LL * c->max_bodies;
+    }
+    // speculative windows need every bead and body of the replica in LDS
+    p.spec = c->mc_window && !p.cache_slice ? 1 : 0;
 }
 
 // Largest workgroups-per-replica count whose grid is co-resident.
@@ -1199,6 +1216,102 @@ int remc_set_state(remc_ctx *c, int replica, const float *x, const float *y, con
     return REMC_OK;
 }
 
+int remc_set_energies(remc_ctx *c, const double *E)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!E) return fail(REMC_ERR_ARG, "null input");
+    const int R = (int)c->reps.size();
+    for (int r = 0; r < R; ++r)
+        if (!std::isfinite(E[r])) return fail(REMC_ERR_ARG, "non-finite energy for replica %d", r);
+    HIP_TRY(hipMemcpyAsync(c->d_E, E, sizeof(double) * R, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_set_counters(remc_ctx *c, const remc_counters *in)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!in) return fail(REMC_ERR_ARG, "null input");
+    const int R = (int)c->reps.size();
+    std::vector<unsigned long long> cnt(3 * R), st(R);
+    for (int r = 0; r < R; ++r) {
+        cnt[3 * r] = in[r].accept;
+        cnt[3 * r + 1] = in[r].accept_boltz;
+        cnt[3 * r + 2] = in[r].reject;
+        st[r] = in[r].steps;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_cnt, cnt.data(), 24 * R, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_step, st.data(), 8 * R, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+static int slot_offset(remc_ctx *c, int sys_id)
+{
+    int off = 0;
+    for (int i = 0; i < sys_id; ++i) off += (int)c->systems[i].replicas.size();
+    return off;
+}
+
+int remc_set_ladder(remc_ctx *c, int sys_id, const int *slot_replica)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size() || !slot_replica) return fail(REMC_ERR_ARG, "bad ladder");
+    const HostSys &s = c->systems[sys_id];
+    const int n = (int)s.replicas.size();
+    std::vector<int> seen(c->reps.size(), 0);
+    for (int k = 0; k < n; ++k) {
+        const int r = slot_replica[k];
+        if (r < 0 || r >= (int)c->reps.size() || c->reps[r].sys != sys_id || seen[r]++)
+            return fail(REMC_ERR_ARG, "ladder entry %d (replica %d) is not a permutation of system %d's replicas", k, r,
+                        sys_id);
+    }
+    const int R = (int)c->reps.size();
+    std::vector<double> T(R);
+    HIP_TRY(hipMemcpyAsync(T.data(), c->d_T, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < n; ++k) T[slot_replica[k]] = s.ladder[k];
+    HIP_TRY(hipMemcpyAsync(c->d_slot + slot_offset(c, sys_id), slot_replica, sizeof(int) * n, hipMemcpyHostToDevice,
+                           c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_T, T.data(), sizeof(double) * R, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_get_exchange_state(remc_ctx *c, int sys_id, uint64_t *round, int *offset, uint64_t *swaps)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "bad system %d", sys_id);
+    unsigned long long rd = 0, sw = 0;
+    int of = 0;
+    HIP_TRY(hipMemcpyAsync(&rd, c->d_exround + sys_id, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&of, c->d_exoff + sys_id, 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(&sw, c->d_exswaps + sys_id, 8, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (round) *round = rd;
+    if (offset) *offset = of;
+    if (swaps) *swaps = sw;
+    return REMC_OK;
+}
+
+int remc_set_exchange_state(remc_ctx *c, int sys_id, uint64_t round, int offset, uint64_t swaps)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "bad system %d", sys_id);
+    if (offset != 0 && offset != 1) return fail(REMC_ERR_ARG, "exchange offset must be 0 or 1");
+    const unsigned long long rd = round, sw = swaps;
+    HIP_TRY(hipMemcpyAsync(c->d_exround + sys_id, &rd, 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_exoff + sys_id, &offset, 4, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->d_exswaps + sys_id, &sw, 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
 int remc_adjoint(remc_ctx *c, int replica, double *force_torque, double *dparams)
 {
     CHECK_CTX(c);
@@ -1293,6 +1406,14 @@ int remc_set_mc_tuning(remc_ctx *c, int lds_cache, int split_min_pairs)
     if (split_min_pairs < 0) return fail(REMC_ERR_ARG, "split threshold must be >= 0");
     c->lds_cache = lds_cache != 0;
     if (split_min_pairs > 0) c->coop_min_pairs = split_min_pairs;
+    destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_set_mc_window(remc_ctx *c, int enable)
+{
+    CHECK_CTX(c);
+    c->mc_window = enable != 0;
     destroy_graph(c);
     return REMC_OK;
 }

@@ -86,6 +86,9 @@ struct KParams {
     int lds_nbody;          // cached bodies per workgroup (0 = none)
     int cache_slice;        // 1: each member caches only its partner slice
     int cache_bodies;       // 1: body ranges and centres cached
+    int cache_rel;          // 1: relative positions cached (full partner cache only)
+    int cache_quat;         // 1: body rotations cached (with the body table)
+    int spec;               // 1: speculative windows of small moves (needs the full caches)
     // recording
     remc_step_record *rec;
     unsigned long long *rec_start;
@@ -95,6 +98,7 @@ struct KParams {
 
 // launchers (remc_kernels.hip)
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
+size_t mc_dynamic_lds(const KParams &p);
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
                               int noncrowder_only, int write_E, hipStream_t s);
@@ -106,6 +110,7 @@ size_t adjoint_ff_size();
 void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh);
 
 constexpr int FULL_TILE = 256;
+constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
 
 } // namespace remc

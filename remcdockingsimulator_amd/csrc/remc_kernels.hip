@@ -130,7 +130,7 @@ __device__ __forceinline__ int pick_kgroups_log2(int nm, int nj)
 // Draws of one MC step and the move parameters that depend on them only
 // (formed ahead of the step, one step per lane).
 struct StepDraw {
-    double u;        // Metropolis uniform
+    double nlu;      // -ln(u) of the Metropolis uniform
     double coef[9];  // rotation coefficients (remc_qcoeffs) of a rotate move
     double q[4];     // rotation quaternion of a rotate move
     float t[3];      // translation vector of a translate move
@@ -165,28 +165,76 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
     return __hiloint2double(hi, lo);
 }
 
-// Dynamic LDS of the MC kernel, carved in this order:
-//   float4 old[mb], float4 new[mb], int tc[mb]           moved body staging
-//   CACHE: float x,y,z,qdh,sig[cl], int tc[cl]            partner beads [cache_lo, cache_lo+cl)
-//   CACHE && cache_bodies: int start,end[nbody]; float cx,cy,cz[nbody]
-// cl, nbody and mb are the launch-wide maxima (KParams::lds_*).
-template <int BLOCK, bool CACHE>
+__device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, const DevSys &S)
+{
+    const float dx = min_image(xj - a.x, S.box, S.inv_box);
+    const float dy = min_image(yj - a.y, S.box, S.inv_box);
+    const float dz = min_image(zj - a.z, S.box, S.inv_box);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
+// Dynamic LDS of the MC kernel, carved in this order (mc_dynamic_lds sizes it):
+//   cache_quat:             double q[4*nbody]                 body rotations
+//   always:                 float4 old[mb], new[mb]; int tc[mb] moved-body staging
+//   CACHE:                  float x,y,z,q,sig[cl]; int tc[cl]  partner beads [clo, clo+cl)
+//   CACHE && cache_rel:     float rx,ry,rz[cl]                 relative positions
+//   CACHE && cache_bodies:  int start,end[nbody]; float cx,cy,cz[nbody]
+// mb, cl and nbody are the launch-wide maxima (KParams::lds_*).
+size_t mc_dynamic_lds(const KParams &p)
+{
+    const size_t mb = p.lds_mb, cl = p.lds_cl, nb = p.lds_nbody;
+    size_t s = 36 * mb;
+    if (p.cache_quat) s += 32 * nb;
+    if (cl) s += 24 * cl;
+    if (cl && p.cache_rel) s += 12 * cl;
+    if (cl && p.cache_bodies) s += 20 * nb;
+    return s;
+}
+
+// One launch runs nsteps MC steps of every replica.  A step whose moved body
+// is small (at most SPEC_MAX beads and not split across members) may be taken
+// in a speculative window when SPEC is set: the next w <= NW such steps are
+// evaluated at once, one wave per step, each against the conformation at the
+// window's start (dE0_k).  The exact dE of step k after the accepted steps
+// j < k of the window differs from dE0_k only by the pair terms between the
+// two moved bodies,
+//   corr(j,k) = sum_{i in k, l in j} [e(n_i,n_l) - e(o_i,n_l)] - [e(n_i,o_l) - e(o_i,o_l)],
+// which are formed for every pair of slots in parallel; one wave then decides
+// the steps in order, adding corr(j,k) to every later slot when step j is
+// accepted.  A step that moves a body an accepted step of the window already
+// moved ends the window (its proposal depends on that move); it starts the
+// next one.  Decisions, energies and states are those of taking the steps one
+// at a time, up to the FP32 rounding of the reassociated pair sums.
+template <int BLOCK, bool CACHE, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
+    constexpr int LB = BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10;
+    constexpr int WS = SPEC ? NW * SPEC_MAX : 1;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
     __shared__ StepDraw s_draw[DRAW_BATCH];
+    __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
+    __shared__ float4 s_wold[WS], s_wnew[WS];
+    __shared__ double s_wbase[SPEC ? NW : 1], s_wcorr[SPEC ? NW * NW : 1], s_wdE[SPEC ? NW : 1];
+    __shared__ int s_wdec[SPEC ? NW : 1];
+    __shared__ int s_weff;
     extern __shared__ float4 smem[];
-    float4 *s_old = smem;
+
+    char *lp = (char *)smem;
+    double *s_q = (double *)lp;
+    lp += p.cache_quat ? 32 * (size_t)p.lds_nbody : 0;
+    float4 *s_old = (float4 *)lp;
     float4 *s_new = s_old + p.lds_mb;
     int *s_tc = (int *)(s_new + p.lds_mb);
     float *s_jx = (float *)(s_tc + p.lds_mb);
     float *s_jy = s_jx + p.lds_cl, *s_jz = s_jy + p.lds_cl, *s_jq = s_jz + p.lds_cl, *s_js = s_jq + p.lds_cl;
     int *s_jtc = (int *)(s_js + p.lds_cl);
-    int *s_bs = s_jtc + p.lds_cl, *s_be = s_bs + p.lds_nbody;
+    const int rel_n = p.cache_rel ? p.lds_cl : 0;
+    float *s_rx = (float *)(s_jtc + p.lds_cl), *s_ry = s_rx + rel_n, *s_rz = s_ry + rel_n;
+    int *s_bs = (int *)(s_rz + rel_n), *s_be = s_bs + p.lds_nbody;
     float *s_cx = (float *)(s_be + p.lds_nbody), *s_cy = s_cx + p.lds_nbody, *s_cz = s_cy + p.lds_nbody;
 
     const int tid = threadIdx.x;
@@ -201,7 +249,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     if (tid == 0) s_abort = 0;
 
     double E = p.E[r];
-    const double T = p.T[r];
+    const double beta = remc_beta(p.T[r]);
     unsigned long long step = p.step[r];
     unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
     const unsigned long long rec_start = p.rec ? p.rec_start[r] : 0ull;
@@ -225,15 +273,22 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     const int clo = (CACHE && p.cache_slice) ? cjs : 0;
     const int chi = CACHE ? ((p.cache_slice) ? cje : N) : 0;
     const bool cache_bodies = CACHE && p.cache_bodies;
+    const bool cache_rel = CACHE && p.cache_rel;
+    const bool cache_quat = p.cache_quat != 0;
     if (CACHE) {
         for (int b = clo + tid; b < chi; b += BLOCK) {
             const int k = b - clo;
             s_jx[k] = gx[b];
             s_jy[k] = gy[b];
             s_jz[k] = gz[b];
-            s_jq[k] = qs[b] * ff.c_dh;
+            s_jq[k] = qs[b];
             s_js[k] = sgs[b];
             s_jtc[k] = tcs[b];
+            if (cache_rel) {
+                s_rx[k] = grx[b];
+                s_ry[k] = gry[b];
+                s_rz[k] = grz[b];
+            }
         }
         if (cache_bodies)
             for (int m = tid; m < S.n_bodies; m += BLOCK) {
@@ -244,32 +299,273 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 s_cz[m] = gcz[m];
             }
     }
+    if (cache_quat)
+        for (int m = tid; m < 4 * S.n_bodies; m += BLOCK) s_q[m] = gq[m];
     __syncthreads();
 
-    for (int it = 0; it < nsteps; ++it, ++step) {
+    int b0 = 0, have = 0; // s_draw holds the draws of steps [b0, b0 + have) of this launch
+    int it = 0;
+    while (it < nsteps) {
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
-        const int slot_d = it & (DRAW_BATCH - 1);
-        if (slot_d == 0) {
+        int off = it - b0;
+        if (off >= have || (SPEC && off + NW > have && b0 + have < nsteps)) {
             // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
-            if (wave == 0 && it + lane < nsteps) {
-                const remc_draws dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
-                StepDraw &sd = s_draw[lane];
-                sd.u = dd.u;
-                sd.body = dd.body;
-                sd.rotate = dd.rotate;
-                if (dd.rotate) {
-                    double qq[4], cf[9];
-                    remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
-                    remc_qcoeffs(qq, cf);
-                    for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
-                    for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
-                } else {
-                    remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
+            if (wave == 0) {
+                bool elig = false;
+                if (it + lane < nsteps) {
+                    const remc_draws dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+                    StepDraw &sd = s_draw[lane];
+                    sd.nlu = remc_neglog_u(dd.u);
+                    sd.body = dd.body;
+                    sd.rotate = dd.rotate;
+                    if (dd.rotate) {
+                        double qq[4], cf[9];
+                        remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
+                        remc_qcoeffs(qq, cf);
+                        for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
+                        for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
+                    } else {
+                        remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
+                    }
+                    if (SPEC) {
+                        const int gb = S.body_off + dd.body;
+                        const int nmb = p.bend[gb] - p.bstart[gb];
+                        elig = nmb <= SPEC_MAX &&
+                               !(coop > 1 && (long long)nmb * (long long)(N - nmb) >= (long long)p.coop_min_pairs);
+                    }
+                }
+                if (SPEC) {
+                    const unsigned long long m = __ballot(elig);
+                    if (lane == 0) s_elig = m;
+                }
+            }
+            b0 = it;
+            have = min(DRAW_BATCH, nsteps - it);
+            off = 0;
+            __syncthreads();
+        }
+
+        int w = 0;
+        if (SPEC) {
+            const unsigned long long run = ~(s_elig >> off);
+            w = run == 0ull ? 64 : (int)__builtin_ctzll(run);
+            w = min(w, NW);
+        }
+        if (SPEC && w > 0) {
+            // ---------------- speculative window of w small moves ----------------
+            // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
+            if (tid < NW * SPEC_MAX) {
+                const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
+                if (k < w) {
+                    const StepDraw &Dk = s_draw[off + k];
+                    const int bk = Dk.body;
+                    const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                    if (i < nmk) {
+                        const int b = m0k + i;
+                        const float rx = cache_rel ? s_rx[b] : grx[b];
+                        const float ry = cache_rel ? s_ry[b] : gry[b];
+                        const float rz = cache_rel ? s_rz[b] : grz[b];
+                        float nrx = rx, nry = ry, nrz = rz;
+                        float ncx = s_cx[bk], ncy = s_cy[bk], ncz = s_cz[bk];
+                        if (Dk.rotate) remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
+                        else remc_wrap_centre(ncx, ncy, ncz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
+                        s_wold[tid] = make_float4(s_jx[b], s_jy[b], s_jz[b], s_jq[b] * ff.c_dh);
+                        s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, s_js[b]);
+                    }
                 }
             }
             __syncthreads();
+
+            // pair corrections between the moved bodies of every two slots j < k
+            if (tid < NW * NW) {
+                const int jj = tid / NW, kk = tid % NW;
+                if (jj < kk && kk < w) {
+                    const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
+                    double cv = 0.0;
+                    if (bj != bk) {
+                        const int m0j = s_bs[bj], nmj = s_be[bj] - m0j;
+                        const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                        const bool crow = ((s_jtc[m0j] | s_jtc[m0k]) >> 8) != 0;
+                        float acc = 0.0f, comp = 0.0f;
+                        for (int i = 0; i < nmk; ++i) {
+                            const float4 o = s_wold[kk * SPEC_MAX + i], n = s_wnew[kk * SPEC_MAX + i];
+                            const int ti = s_jtc[m0k + i] & 0xff;
+                            for (int l = 0; l < nmj; ++l) {
+                                const float4 O = s_wold[jj * SPEC_MAX + l], Nn = s_wnew[jj * SPEC_MAX + l];
+                                float e_nn, e_on, e_no, e_oo;
+                                if (crow) {
+                                    e_nn = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, n, S), ff);
+                                    e_on = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, o, S), ff);
+                                    e_no = pair_crowder(dist2(O.x, O.y, O.z, n, S), ff);
+                                    e_oo = pair_crowder(dist2(O.x, O.y, O.z, o, S), ff);
+                                } else {
+                                    const int bl = m0j + l;
+                                    const float eij = s_etab[ti * REMC_NTYPES + (s_jtc[bl] & 0xff)];
+                                    const float sij = 0.5f * (n.w + s_js[bl]);
+                                    const float sij2 = sij * sij;
+                                    const float qq = o.w * s_jq[bl];
+                                    e_nn = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, n, S), eij, sij2, qq, ff);
+                                    e_on = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, o, S), eij, sij2, qq, ff);
+                                    e_no = pair_lj_dh(dist2(O.x, O.y, O.z, n, S), eij, sij2, qq, ff);
+                                    e_oo = pair_lj_dh(dist2(O.x, O.y, O.z, o, S), eij, sij2, qq, ff);
+                                }
+                                kahan_add(acc, comp, (e_nn - e_on) - (e_no - e_oo));
+                            }
+                        }
+                        cv = (double)acc - (double)comp;
+                    }
+                    s_wcorr[jj * NW + kk] = cv;
+                }
+            }
+            // dE0 of slot k = wave k: its moved beads against every other bead
+            if (wave < w) {
+                const int k = wave;
+                const int bk = s_draw[off + k].body;
+                const int m0k = s_bs[bk], m1k = s_be[bk];
+                const bool crow_m = (s_jtc[m0k] >> 8) != 0;
+                float acc = 0.0f, comp = 0.0f;
+                for (int i = 0; i < m1k - m0k; ++i) {
+                    const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
+                    const float *erow = s_etab + (s_jtc[m0k + i] & 0xff) * REMC_NTYPES;
+                    for (int j = lane; j < N; j += 64) {
+                        if (j >= m0k && j < m1k) continue;
+                        const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
+                        const int tcj = s_jtc[j];
+                        float eo, en;
+                        if (crow_m || (tcj >> 8) != 0) {
+                            eo = pair_crowder(dist2(xj, yj, zj, o, S), ff);
+                            en = pair_crowder(dist2(xj, yj, zj, n, S), ff);
+                        } else {
+                            const float eij = erow[tcj & 0xff];
+                            const float sij = 0.5f * (n.w + s_js[j]);
+                            const float sij2 = sij * sij;
+                            const float qq = o.w * s_jq[j];
+                            eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
+                            en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
+                        }
+                        kahan_add(acc, comp, en - eo);
+                    }
+                }
+                const double v = wave_sum_dpp((double)acc - (double)comp);
+                if (lane == 0) s_wbase[k] = v;
+            }
+            __syncthreads();
+
+            // decisions in step order (wave 0); lane l carries slot l's dE
+            if (wave == 0) {
+                double dEl = lane < w ? s_wbase[lane] : 0.0;
+                const int bl = lane < w ? s_draw[off + lane].body : -1;
+                unsigned long long accm = 0ull;
+                int weff = w;
+                for (int k = 0; k < w; ++k) {
+                    const int bk = __shfl(bl, k, 64);
+                    const bool moved = lane < k && ((accm >> lane) & 1ull) && bl == bk;
+                    if (__ballot(moved) != 0ull) {
+                        weff = k;
+                        break;
+                    }
+                    const double dEk = __shfl(dEl, k, 64);
+                    const int dec = remc_metropolis_l(dEk, beta, s_draw[off + k].nlu);
+                    if (lane == 0) {
+                        s_wdec[k] = dec;
+                        s_wdE[k] = dEk;
+                    }
+                    if (dec) {
+                        accm |= 1ull << k;
+                        if (lane > k && lane < w) dEl += s_wcorr[k * NW + lane];
+                    }
+                }
+                if (lane == 0) s_weff = weff;
+            }
+            __syncthreads();
+            const int weff = s_weff;
+
+            // commit the accepted moves (their bodies are distinct)
+            if (tid < NW * SPEC_MAX) {
+                const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
+                if (k < weff && s_wdec[k]) {
+                    const StepDraw &Dk = s_draw[off + k];
+                    const int bk = Dk.body;
+                    const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                    if (i < nmk) {
+                        const int b = m0k + i;
+                        const float4 n = s_wnew[tid];
+                        gx[b] = n.x;
+                        gy[b] = n.y;
+                        gz[b] = n.z;
+                        s_jx[b] = n.x;
+                        s_jy[b] = n.y;
+                        s_jz[b] = n.z;
+                        if (Dk.rotate) {
+                            const float rx = cache_rel ? s_rx[b] : grx[b];
+                            const float ry = cache_rel ? s_ry[b] : gry[b];
+                            const float rz = cache_rel ? s_rz[b] : grz[b];
+                            float nrx, nry, nrz;
+                            remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
+                            grx[b] = nrx;
+                            gry[b] = nry;
+                            grz[b] = nrz;
+                            if (cache_rel) {
+                                s_rx[b] = nrx;
+                                s_ry[b] = nry;
+                                s_rz[b] = nrz;
+                            }
+                        }
+                        if (i == 0) {
+                            if (Dk.rotate) {
+                                double qo[4], qn[4];
+                                for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * bk + t] : gq[4 * bk + t];
+                                remc_qmul(Dk.q, qo, qn);
+                                for (int t = 0; t < 4; ++t) {
+                                    gq[4 * bk + t] = qn[t];
+                                    if (cache_quat) s_q[4 * bk + t] = qn[t];
+                                }
+                            } else {
+                                float ncx, ncy, ncz;
+                                remc_wrap_centre(s_cx[bk], s_cy[bk], s_cz[bk], Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx,
+                                                 &ncy, &ncz);
+                                gcx[bk] = ncx;
+                                gcy[bk] = ncy;
+                                gcz[bk] = ncz;
+                                s_cx[bk] = ncx;
+                                s_cy[bk] = ncy;
+                                s_cz[bk] = ncz;
+                            }
+                        }
+                    }
+                }
+            }
+            for (int k = 0; k < weff; ++k) {
+                const int dec = s_wdec[k];
+                if (dec) {
+                    E += s_wdE[k];
+                    if (dec == 1) ++n_acc; else ++n_accb;
+                } else {
+                    ++n_rej;
+                }
+            }
+            if (c == 0 && p.rec && tid < weff) {
+                const unsigned long long st = step + tid;
+                const unsigned long long idx = st - rec_start;
+                if (st >= rec_start && idx < (unsigned long long)p.rec_max) {
+                    const StepDraw &Dk = s_draw[off + tid];
+                    remc_step_record rr;
+                    rr.body = Dk.body;
+                    rr.move = Dk.rotate ? 1 : 0;
+                    rr.accepted = s_wdec[tid];
+                    rr.ticks = (int32_t)((wall_clock64() - t_step) / (unsigned long long)weff);
+                    rr.delta = s_wdE[tid];
+                    p.rec[(size_t)r * p.rec_max + idx] = rr;
+                }
+            }
+            __syncthreads();
+            it += weff;
+            step += weff;
+            continue;
         }
-        const StepDraw &D = s_draw[slot_d];
+
+        // ---------------------------- one step ----------------------------
+        const StepDraw &D = s_draw[off];
         const int body = D.body;
         const bool rotate = D.rotate != 0;
         int m0, m1;
@@ -289,19 +585,23 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             cz = gcz[body];
         }
         const int nm = m1 - m0;
-        const bool crow_m = p.bcrow[S.body_off + body] != 0;
+        const bool crow_m = (CACHE && m0 >= clo && m0 < chi) ? (s_jtc[m0 - clo] >> 8) != 0
+                                                             : p.bcrow[S.body_off + body] != 0;
         float ncx = cx, ncy = cy, ncz = cz;
         if (!rotate) remc_wrap_centre(cx, cy, cz, D.t[0], D.t[1], D.t[2], S.box, &ncx, &ncy, &ncz);
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
-            float rx = grx[b], ry = gry[b], rz = grz[b];
+            const bool inc = CACHE && b >= clo && b < chi;
+            const float rx = cache_rel ? s_rx[b] : grx[b];
+            const float ry = cache_rel ? s_ry[b] : gry[b];
+            const float rz = cache_rel ? s_rz[b] : grz[b];
             float nrx = rx, nry = ry, nrz = rz;
             if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
-            const bool inc = CACHE && b >= clo && b < chi;
-            s_old[k] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qs[b])
-                           : make_float4(gx[b], gy[b], gz[b], qs[b]);
-            s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
-            s_tc[k] = tcs[b];
+            const float qb = inc ? s_jq[b - clo] : qs[b];
+            s_old[k] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
+                           : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
+            s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+            s_tc[k] = inc ? s_jtc[b - clo] : tcs[b];
         }
         __syncthreads();
 
@@ -313,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         // split into tk groups of moved beads x G lanes over partner beads
         const int ov = max(0, min(je, m1) - max(js, m0));
         const int lt = pick_kgroups_log2<BLOCK>(nm, (je - js) - ov);
-        const int lgw = (BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10) - lt;
+        const int lgw = LB - lt;
         const int G = 1 << lgw;
         const int g = tid >> lgw, l = tid & (G - 1);
         const int k0 = (g * nm) >> lt, k1 = ((g + 1) * nm) >> lt;
@@ -334,22 +634,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 xj = gx[j];
                 yj = gy[j];
                 zj = gz[j];
-                qj = qs[j] * ff.c_dh;
+                qj = qs[j];
                 sj = sgs[j];
                 tcj = tcs[j];
             }
             const bool cj = (tcj >> 8) != 0;
             if (crow_m || cj) {
                 for (int k = k0; k < k1; ++k) {
-                    const float4 o = s_old[k], n = s_new[k];
-                    float dx = min_image(xj - o.x, S.box, S.inv_box);
-                    float dy = min_image(yj - o.y, S.box, S.inv_box);
-                    float dz = min_image(zj - o.z, S.box, S.inv_box);
-                    float eo = pair_crowder(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), ff);
-                    dx = min_image(xj - n.x, S.box, S.inv_box);
-                    dy = min_image(yj - n.y, S.box, S.inv_box);
-                    dz = min_image(zj - n.z, S.box, S.inv_box);
-                    float en = pair_crowder(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), ff);
+                    const float eo = pair_crowder(dist2(xj, yj, zj, s_old[k], S), ff);
+                    const float en = pair_crowder(dist2(xj, yj, zj, s_new[k], S), ff);
                     kahan_add(acc, comp, en - eo);
                 }
             } else {
@@ -361,14 +654,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     const float sij = 0.5f * (n.w + sj);
                     const float sij2 = sij * sij;
                     const float qq = o.w * qj;
-                    float dx = min_image(xj - o.x, S.box, S.inv_box);
-                    float dy = min_image(yj - o.y, S.box, S.inv_box);
-                    float dz = min_image(zj - o.z, S.box, S.inv_box);
-                    float eo = pair_lj_dh(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), eij, sij2, qq, ff);
-                    dx = min_image(xj - n.x, S.box, S.inv_box);
-                    dy = min_image(yj - n.y, S.box, S.inv_box);
-                    dz = min_image(zj - n.z, S.box, S.inv_box);
-                    float en = pair_lj_dh(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), eij, sij2, qq, ff);
+                    const float eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
+                    const float en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
                     kahan_add(acc, comp, en - eo);
                 }
             }
@@ -379,26 +666,26 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         double dE = 0.0;
         if (!split) {
 #pragma unroll
-            for (int w = 0; w < NW; ++w) dE += s_red[w];
+            for (int w2 = 0; w2 < NW; ++w2) dE += s_red[w2];
         } else {
             if (wave == 0) {
                 double part = 0.0;
 #pragma unroll
-                for (int w = 0; w < NW; ++w) part += s_red[w];
+                for (int w2 = 0; w2 < NW; ++w2) part += s_red[w2];
                 // slots alternate with the split-step count: a member reuses a
                 // slot only after every member has read it (they all wrote
                 // their partial of the split step in between)
                 const unsigned long long tag = step & 0x7FFFFull;
                 unsigned long long *slot = slots + (n_split & 1u) * REMC_MAX_COOP;
                 if (lane == 0) __hip_atomic_store(slot + c, slot_pack(part, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                unsigned long long w = 0ull;
+                unsigned long long wv = 0ull;
                 bool ok = lane >= coop;
                 int spins = 0;
                 bool failed = false;
                 while (!__all(ok)) {
                     if (!ok) {
-                        w = __hip_atomic_load(slot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                        ok = (w & SLOT_TAG_MASK) == tag;
+                        wv = __hip_atomic_load(slot + lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        ok = (wv & SLOT_TAG_MASK) == tag;
                     }
                     if (++spins >= SLOT_SPIN_LIMIT) {
                         failed = true;
@@ -418,7 +705,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 } else {
                     // members' partials summed in member order: identical in every member
                     double tot = 0.0;
-                    for (int m = 0; m < coop; ++m) tot += slot_value(__shfl(w, m, 64));
+                    for (int m = 0; m < coop; ++m) tot += slot_value(__shfl(wv, m, 64));
                     if (lane == 0) s_dE = tot;
                 }
             }
@@ -428,7 +715,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             ++n_split;
         }
 
-        const int decision = remc_metropolis(dE, T, D.u);
+        const int decision = remc_metropolis_l(dE, beta, D.nlu);
         if (decision) {
             for (int k = tid; k < nm; k += BLOCK) {
                 const int b = m0 + k;
@@ -442,23 +729,30 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     s_jz[b - clo] = n.z;
                 }
                 if (rotate) {
+                    const float rx = cache_rel ? s_rx[b] : grx[b];
+                    const float ry = cache_rel ? s_ry[b] : gry[b];
+                    const float rz = cache_rel ? s_rz[b] : grz[b];
                     float nrx, nry, nrz;
-                    remc_qrotate(D.coef, grx[b], gry[b], grz[b], &nrx, &nry, &nrz);
+                    remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
                     grx[b] = nrx;
                     gry[b] = nry;
                     grz[b] = nrz;
+                    if (cache_rel) {
+                        s_rx[b] = nrx;
+                        s_ry[b] = nry;
+                        s_rz[b] = nrz;
+                    }
                 }
             }
             if (tid == 0) {
                 if (rotate) {
-                    double qo[4] = {gq[4 * body + 0], gq[4 * body + 1], gq[4 * body + 2], gq[4 * body + 3]};
-                    double qm[4] = {D.q[0], D.q[1], D.q[2], D.q[3]};
-                    double qn[4];
-                    remc_qmul(qm, qo, qn);
-                    gq[4 * body + 0] = qn[0];
-                    gq[4 * body + 1] = qn[1];
-                    gq[4 * body + 2] = qn[2];
-                    gq[4 * body + 3] = qn[3];
+                    double qo[4], qn[4];
+                    for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * body + t] : gq[4 * body + t];
+                    remc_qmul(D.q, qo, qn);
+                    for (int t = 0; t < 4; ++t) {
+                        gq[4 * body + t] = qn[t];
+                        if (cache_quat) s_q[4 * body + t] = qn[t];
+                    }
                 } else {
                     gcx[body] = ncx;
                     gcy[body] = ncy;
@@ -488,6 +782,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             }
         }
         __syncthreads();
+        ++it;
+        ++step;
     }
     if (c == 0 && tid == 0 && !s_abort) {
         p.E[r] = E;
@@ -504,59 +800,77 @@ __global__ void k_reset_slots(unsigned long long *slots, int n)
     if (i < n) slots[i] = ~0ull;
 }
 
-size_t mc_dynamic_lds(const KParams &p)
-{
-    return sizeof(float4) * 2 * (size_t)p.lds_mb + sizeof(int) * (size_t)p.lds_mb +
-           (sizeof(float) * 5 + sizeof(int)) * (size_t)p.lds_cl + (sizeof(int) * 2 + sizeof(float) * 3) * (size_t)p.lds_nbody;
-}
-
-template <int BLOCK, bool CACHE>
+template <int BLOCK, bool CACHE, bool SPEC>
 static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, hipStream_t s)
 {
     const size_t lds = mc_dynamic_lds(p);
-    hipError_t e = hipFuncSetAttribute((const void *)k_mc<BLOCK, CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void *)k_mc<BLOCK, CACHE, SPEC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_mc<BLOCK, CACHE>), dim3(grid), dim3(BLOCK), lds, s, p, nsteps, coop);
+    hipLaunchKernelGGL((k_mc<BLOCK, CACHE, SPEC>), dim3(grid), dim3(BLOCK), lds, s, p, nsteps, coop);
     return hipGetLastError();
+}
+
+// kernel variant: 0 = no partner cache, 1 = partner cache, 2 = speculative windows
+static int mc_variant(const KParams &p) { return p.lds_cl <= 0 ? 0 : p.spec ? 2 : 1; }
+
+template <int BLOCK>
+static hipError_t launch_mc_b(const KParams &p, int grid, int nsteps, int coop, hipStream_t s)
+{
+    switch (mc_variant(p)) {
+    case 0: return launch_mc_t<BLOCK, false, false>(p, grid, nsteps, coop, s);
+    case 1: return launch_mc_t<BLOCK, true, false>(p, grid, nsteps, coop, s);
+    default: return launch_mc_t<BLOCK, true, true>(p, grid, nsteps, coop, s);
+    }
 }
 
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s)
 {
     if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
     if (coop < 1 || coop > REMC_MAX_COOP) return hipErrorInvalidValue;
+    // the speculative kernel indexes its caches by bead and body: they must hold the whole replica
+    if (p.spec && (p.lds_cl <= 0 || p.cache_slice || !p.cache_bodies)) return hipErrorInvalidValue;
     if (coop > 1) {
         const int n = n_rep * 2 * REMC_MAX_COOP;
         hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
     }
     const int grid = n_rep * coop;
-    const bool cache = p.lds_cl > 0;
     switch (block) {
-    case 256: return cache ? launch_mc_t<256, true>(p, grid, nsteps, coop, s) : launch_mc_t<256, false>(p, grid, nsteps, coop, s);
-    case 512: return cache ? launch_mc_t<512, true>(p, grid, nsteps, coop, s) : launch_mc_t<512, false>(p, grid, nsteps, coop, s);
-    case 1024: return cache ? launch_mc_t<1024, true>(p, grid, nsteps, coop, s) : launch_mc_t<1024, false>(p, grid, nsteps, coop, s);
+    case 256: return launch_mc_b<256>(p, grid, nsteps, coop, s);
+    case 512: return launch_mc_b<512>(p, grid, nsteps, coop, s);
+    case 1024: return launch_mc_b<1024>(p, grid, nsteps, coop, s);
     default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BLOCK, bool CACHE, bool SPEC>
+static hipError_t occupancy_t(const KParams &p, int *out)
+{
+    const size_t lds = mc_dynamic_lds(p);
+    const void *f = (const void *)k_mc<BLOCK, CACHE, SPEC>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<BLOCK, CACHE, SPEC>, BLOCK, lds);
+}
+
+template <int BLOCK>
+static hipError_t occupancy_b(const KParams &p, int *out)
+{
+    switch (mc_variant(p)) {
+    case 0: return occupancy_t<BLOCK, false, false>(p, out);
+    case 1: return occupancy_t<BLOCK, true, false>(p, out);
+    default: return occupancy_t<BLOCK, true, true>(p, out);
     }
 }
 
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out)
 {
-    const size_t lds = mc_dynamic_lds(p);
-    const bool cache = p.lds_cl > 0;
-#define REMC_OCC(B)                                                                                                    \
-    do {                                                                                                               \
-        const void *f = cache ? (const void *)k_mc<B, true> : (const void *)k_mc<B, false>;                            \
-        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                   \
-        if (e != hipSuccess) return e;                                                                                 \
-        return cache ? hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<B, true>, B, lds)                        \
-                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc<B, false>, B, lds);                      \
-    } while (0)
     switch (block) {
-    case 256: REMC_OCC(256);
-    case 512: REMC_OCC(512);
-    case 1024: REMC_OCC(1024);
+    case 256: return occupancy_b<256>(p, out);
+    case 512: return occupancy_b<512>(p, out);
+    case 1024: return occupancy_b<1024>(p, out);
     default: return hipErrorInvalidValue;
     }
-#undef REMC_OCC
 }
 
 // --------------------------------------------------------------------------

@@ -215,13 +215,66 @@ REMC_HD void remc_move_centre(double a, double b, double c, float step, float bo
     remc_wrap_centre(cx, cy, cz, tx, ty, tz, box, px, py, pz);
 }
 
-// Metropolis test in FP64 (Replica.cpp:438-459): 1 = accept (dE < 0),
-// 2 = Boltzmann accept, 0 = reject.
-REMC_HD int remc_metropolis(double delta, double temperature, double u)
+// -ln(u) in FP64 for a uniform u in [0, 1): u = m 2^e with m in
+// [sqrt(1/2), sqrt(2)), ln m = 2 atanh(s), s = (m-1)/(m+1), |s| <= 0.1716,
+// the odd series of atanh to s^23 (truncation below 1e-19 relative), and
+// e ln2 with the fdlibm split of ln2.  u = 0 maps to the largest x with
+// exp(-x) > 0 in FP64, so that the Metropolis test below accepts exactly the
+// moves u < exp(-x) accepts when u = 0.
+REMC_HD double remc_neglog_u(double u)
+{
+    if (!(u > 0.0)) return 745.1332191019412;
+    int64_t bits;
+#if defined(__HIP_DEVICE_COMPILE__)
+    bits = __double_as_longlong(u);
+#else
+    memcpy(&bits, &u, sizeof bits);
+#endif
+    int e = (int)((bits >> 52) & 0x7ff) - 1023; // u >= 2^-32 here: never subnormal
+    int64_t mb = (bits & 0x000FFFFFFFFFFFFFLL) | 0x3FF0000000000000LL;
+    double m;
+#if defined(__HIP_DEVICE_COMPILE__)
+    m = __longlong_as_double(mb);
+#else
+    memcpy(&m, &mb, sizeof m);
+#endif
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = fma(p, s2, 1.0 / 21.0);
+    p = fma(p, s2, 1.0 / 19.0);
+    p = fma(p, s2, 1.0 / 17.0);
+    p = fma(p, s2, 1.0 / 15.0);
+    p = fma(p, s2, 1.0 / 13.0);
+    p = fma(p, s2, 1.0 / 11.0);
+    p = fma(p, s2, 1.0 / 9.0);
+    p = fma(p, s2, 1.0 / 7.0);
+    p = fma(p, s2, 1.0 / 5.0);
+    p = fma(p, s2, 1.0 / 3.0);
+    const double lm = fma(2.0 * s * s2, p, 2.0 * s);
+    const double ln2_hi = 6.93147180369123816490e-01;
+    const double ln2_lo = 1.90821492927058770002e-10;
+    return -(fma((double)e, ln2_hi, fma((double)e, ln2_lo, lm)));
+}
+
+// Metropolis test in FP64 (Replica.cpp:438-459) with the uniform given as
+// L = -ln(u): 1 = accept (dE < 0), 2 = Boltzmann accept (u < exp(-dE beta),
+// i.e. dE beta < L), 0 = reject.  beta = 4184 / (R T) in mol/kcal.
+REMC_HD int remc_metropolis_l(double delta, double beta, double L)
 {
     if (delta < 0.0) return 1;
-    double beta = 4184.0 / (REMC_R_GAS * temperature);
-    return u < remc_exp(-delta * beta) ? 2 : 0;
+    return delta * beta < L ? 2 : 0;
+}
+
+REMC_HD double remc_beta(double temperature) { return 4184.0 / (REMC_R_GAS * temperature); }
+
+REMC_HD int remc_metropolis(double delta, double temperature, double u)
+{
+    return remc_metropolis_l(delta, remc_beta(temperature), remc_neglog_u(u));
 }
 
 // Exchange test of ladder rungs (Ti, Tj) holding energies (Ei, Ej)

@@ -165,6 +165,10 @@ class Engine:
         self._chk(self.lib.remc_set_mc_tuning(self.ctx, 1 if lds_cache else 0, int(split_min_pairs)),
                   "remc_set_mc_tuning")
 
+    def set_mc_window(self, enable=True):
+        """Speculative windows of small moves in the MC kernel (default on)."""
+        self._chk(self.lib.remc_set_mc_window(self.ctx, 1 if enable else 0), "remc_set_mc_window")
+
     def adjoint(self, replica):
         m = self.systems[self.replica_sys[replica]].n_bodies
         ft = np.zeros((m, 6), dtype=np.float64)
@@ -212,6 +216,32 @@ class Engine:
         qa = None if quat is None else np.ascontiguousarray(quat, dtype=np.float64).reshape(-1)
         self._chk(self.lib.remc_set_state(self.ctx, replica, *ptrs, nat.dptr(qa) if qa is not None else None),
                   "remc_set_state")
+
+    # ------------------------------------------------------ checkpoint ----
+    def set_energies(self, energies):
+        e = np.ascontiguousarray(energies, dtype=np.float64)
+        self._chk(self.lib.remc_set_energies(self.ctx, nat.dptr(e)), "remc_set_energies")
+
+    def set_counters(self, counters):
+        """counters: (n_replicas, 4) accept, accept_boltz, reject, RNG steps."""
+        arr = (nat.remc_counters * self.n_replicas)()
+        for r, row in enumerate(np.asarray(counters, dtype=np.uint64)):
+            arr[r].accept, arr[r].accept_boltz, arr[r].reject, arr[r].steps = (int(v) for v in row)
+        self._chk(self.lib.remc_set_counters(self.ctx, arr), "remc_set_counters")
+
+    def set_ladder(self, sys_id, slot_replica):
+        a = np.ascontiguousarray(slot_replica, dtype=np.int32)
+        self._chk(self.lib.remc_set_ladder(self.ctx, sys_id, nat.iptr(a)), "remc_set_ladder")
+
+    def exchange_state(self, sys_id):
+        rd, sw, of = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_int()
+        self._chk(self.lib.remc_get_exchange_state(self.ctx, sys_id, ctypes.byref(rd), ctypes.byref(of),
+                                                   ctypes.byref(sw)), "remc_get_exchange_state")
+        return rd.value, of.value, sw.value
+
+    def set_exchange_state(self, sys_id, round_, offset, swaps):
+        self._chk(self.lib.remc_set_exchange_state(self.ctx, sys_id, int(round_), int(offset), int(swaps)),
+                  "remc_set_exchange_state")
 
     # ---------------------------------------------------------- timing ----
     MC, FULL, PLAIN, EXCHANGE, ADJOINT = range(5)

@@ -89,3 +89,13 @@ def test_temperature_swap_and_thread_clamp(tmp_path, aas):
     assert "swapping" in r.stdout
     assert "maximum temperature 400" in r.stdout and "minimum temperature 280" in r.stdout
     assert "Too many threads, setting equal to 4" in r.stdout
+
+
+def test_resume_rejects_missing_or_malformed_checkpoint(tmp_path):
+    tmp = str(tmp_path)
+    assert run_sim(["-z", "nope", "-q"], cwd=tmp).returncode != 0
+    bad = os.path.join(tmp, "bad")
+    with open(bad, "w") as f:
+        f.write("#Saved simulation state file\ncfg\nx_1\n1\n# p\n1 2 3\n")
+    r = run_sim(["-z", bad, "-q"], cwd=tmp)
+    assert r.returncode != 0 and "malformed" in r.stdout

@@ -1,0 +1,153 @@
+"""remc_sim end to end on the device: a short REMC run of the reference's
+UIM/ubiquitin complex (conformation fixture 1, both chains left in their
+docked pose) with two crowders, checked against the reference's output
+formats ("simulation engine/src/main.cpp:634-713,1378-1521",
+Replica.cpp:1717-1780) and, record by record, against the oracle: every
+bound-conformation record's poses are rebuilt from the printed quaternions
+and centres and its printed energies recomputed in FP64."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from facade_helpers import DATA, python_molecules, run_sim, split_chains, write_config
+from helpers import say
+from remcdockingsimulator_amd import forcefield as ffm
+from remcdockingsimulator_amd import model
+
+pytestmark = pytest.mark.gpu
+
+PLACE = [
+    ("t", (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, -1.0), "A", False),
+    ("t", (0.0, 0.0, 0.0), (1.0, 0.0, 0.0, -1.0), "B", False),
+    ("p", (100.0, 100.0, 100.0), (0.0, 1.0, 0.0, 0.29), "A", True),
+    ("p", (120.0, 60.0, 90.0), (1.0, 1.0, 0.0, 0.5), "A", True),
+]
+PARAMS = {"mcsteps": 2000, "resteps": 4, "replicas": 4, "samplefrequency": 100, "sampleafter": 0,
+          "boundry": 150, "temperaturemax": 420, "temperaturemin": 290, "prefix": "g1"}
+
+
+def parse_bound(path):
+    recs = []
+    head = re.compile(r"^(\d+); (-?[\d.]+|-?nan|-?inf) \((-?[\d.]+)\); ([\d.]+)$")
+    with open(path) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    assert lines[0] == "# iteration; complex free energy (replica free energy); temperature;"
+    assert lines[1].startswith("# molecule: rotation(w,x,y,z) position(x,y,z)")
+    i = 2
+    while i < len(lines):
+        m = head.match(lines[i])
+        assert m, lines[i]
+        poses = []
+        for a in range(len(PLACE)):
+            t = lines[i + 1 + a].split()
+            assert t[0] == ("0" if a < 2 else "1") and t[1] == "%d:" % a, lines[i + 1 + a]
+            poses.append([float(v) for v in t[2:]])
+        recs.append((int(m.group(1)), float(m.group(2)), float(m.group(3)), float(m.group(4)), np.array(poses)))
+        i += 1 + len(PLACE)
+    return recs
+
+
+def test_remc_sim_short_run(tmp_path, oracle, aas):
+    tmp = str(tmp_path)
+    paths = split_chains(1, tmp)
+    cfg = os.path.join(tmp, "cfg")
+    write_config(cfg, PARAMS, PLACE)
+    say("remc_sim short run")
+    r = run_sim(["-f", cfg, "--seed", "5", "--outdir", "out", "--pid", "9", "--final-pdb", "final.pdb"], cwd=tmp)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    out = os.path.join(tmp, "out")
+    pre = os.path.join(out, "g1_repulsive_9_")
+
+    T = 290.0 * (420.0 / 290.0) ** (np.arange(4) / 3.0)
+    fb = open(pre + "fractionBound").read().splitlines()
+    assert fb[:2] == ["# Fraction Bound", "#Iteration InstantaniousAve CumulativeAve"]
+    assert fb[2] == "Iteration:  " + "".join("%0.1fKi %0.1fKc " % (np.float32(t), np.float32(t)) for t in T) + "\t"
+    rows = fb[3:]
+    assert [int(x.split(":")[0]) for x in rows] == [500, 1000, 1500, 2000]
+    for x in rows:
+        vals = [float(v) for v in re.findall(r"\| (\S+) (\S+)", x) for v in v]
+        assert len(vals) == 8 and all(0.0 <= v <= 1.0 for v in vals), x
+
+    ar = open(pre + "acceptance_ratios").read().splitlines()
+    assert ar[0] == "# Iteration AcceptanceRatio" and ar[1].startswith("Iteration:  ")
+    assert [int(x.split(":")[0]) for x in ar[2:]] == [500, 1000, 1500, 2000]
+
+    ex = open(pre + "exchange_freq").read().splitlines()
+    assert ex[0] == "# Iteration continous_ratio, inst_ratio"
+    assert len(ex) == 4
+    for k, x in enumerate(ex[1:], start=1):
+        assert re.match(r"^ {9}%d \d\.\d{4} \d\.\d{4}$" % k, x), x
+
+    idx = open(pre + "fileindex").read()
+    assert idx.endswith(" 2 seg_A.pdb Y\n 3 seg_A.pdb Y\n")
+
+    # every bound record against the oracle
+    recs = parse_bound(pre + "boundconformations")
+    say("bound records", len(recs))
+    assert recs, "the docked complex should be sampled bound at least once"
+    mols = python_molecules(paths, aas, PLACE)
+    sa0 = model.pack_molecules(mols, 150.0)
+    ff = ffm.ForceField.mj1996(DATA)
+    osys = oracle.OracleSystem(sa0, ff)
+    body = np.repeat(np.arange(sa0.n_bodies), np.diff(sa0.body_start))
+    rel0 = sa0.rel.astype(np.float64)
+    for step, complex_e, pot, temp, poses in recs:
+        assert step % 100 == 0 and 0 <= step < 2000
+        assert any(abs(temp - float("%0.1f" % np.float32(t))) < 1e-9 for t in T)
+        pos = np.empty_like(rel0)
+        for a in range(sa0.n_bodies):
+            q = poses[a, :4]
+            sel = body == a
+            pos[sel] = model.quat_rotate(q, rel0[sel].astype(np.float32)).astype(np.float64) + poses[a, 4:]
+        e_full = osys.energy_d(pos)
+        e_nc = osys.energy_noncrowder(pos.astype(np.float32))
+        assert abs(pot - e_full) <= 2e-3 + 1e-4 * abs(e_full), (step, pot, e_full)
+        assert abs(complex_e - (e_full + e_nc)) <= 4e-3 + 1e-4 * abs(e_full + e_nc), (step, complex_e, e_full + e_nc)
+        assert complex_e < model_bound_threshold()
+
+    pdb = open(os.path.join(tmp, "final.pdb")).read()
+    assert pdb.count("\nATOM  ") + pdb.startswith("ATOM") == sa0.n_beads and pdb.endswith("END \n")
+
+
+def model_bound_threshold():
+    return -1.1844  # BOUND_ENERGY_VALUE (definitions.h:84)
+
+
+def test_remc_sim_checkpoint_resume_is_exact(tmp_path, aas):
+    """A run that checkpoints after exchange round 2 of 4, and a second process
+    resumed from that checkpoint, end in the same state and write the same
+    output lines for rounds 3 and 4 (main.cpp:89-122 intent, working load)."""
+    tmp = str(tmp_path)
+    split_chains(3, tmp)
+    cfg = os.path.join(tmp, "cfg")
+    write_config(cfg, PARAMS, PLACE)
+    common = ["-bx", "1024", "--coop", "1", "--checkpoint-dir", "ck"]
+    say("remc_sim run with a checkpoint")
+    a = run_sim(["-f", cfg, "--seed", "11", "--outdir", "outA", "--pid", "21", "--checkpoint", "1000",
+                 "--final-pdb", "finalA.pdb"] + common, cwd=tmp)
+    assert a.returncode == 0, a.stdout[-3000:] + a.stderr[-3000:]
+    ckf = os.path.join(tmp, "ck", "g1_repulsive_21")
+    assert os.path.exists(ckf)
+    say("remc_sim resumed from the checkpoint")
+    b = run_sim(["-z", ckf, "--outdir", "outB", "--final-pdb", "finalB.pdb"] + common, cwd=tmp)
+    assert b.returncode == 0, b.stdout[-3000:] + b.stderr[-3000:]
+    assert "Resuming after exchange round 2" in b.stdout
+
+    def body(path):
+        return open(os.path.join(tmp, path)).read().splitlines()[1:]  # drop the REMARK with the file name
+
+    assert body("finalA.pdb") == body("finalB.pdb")
+    pa, pb = (os.path.join(tmp, d, "g1_repulsive_21_") for d in ("outA", "outB"))
+    for name in ("fractionBound", "acceptance_ratios"):
+        la = open(pa + name).read().splitlines()
+        lb = open(pb + name).read().splitlines()
+        assert lb == [x for x in la if re.match(r"^ +(1500|2000):", x)], name
+    ea = open(pa + "exchange_freq").read().splitlines()
+    eb = open(pb + "exchange_freq").read().splitlines()
+    assert eb == ea[-1:]
+    ba = open(pa + "boundconformations").read().splitlines()[2:]
+    bb = open(pb + "boundconformations").read().splitlines()
+    cut = next((i for i, x in enumerate(ba) if re.match(r"^\d+;", x) and int(x.split(";")[0]) >= 1000), len(ba))
+    assert bb == ba[cut:]

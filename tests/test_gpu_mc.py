@@ -19,13 +19,15 @@ pytestmark = pytest.mark.gpu
 SEED = 0xC0FFEE
 
 
-def run_parity(oracle, cid, nsteps, block=None, replicas=None, coop=None):
+def run_parity(oracle, cid, nsteps, block=None, replicas=None, coop=None, window=True):
     wl = synth.config(cid)
     sa = wl.systems[0]
     lad = wl.temperatures[0]
     e = engine_for(wl, seed=SEED, block=block)
     if coop:
         e.set_mc_coop(coop)
+    if not window:
+        e.set_mc_window(False)
     e.set_recording(nsteps)
     e.mc_steps(nsteps)
     e.synchronize()
@@ -79,6 +81,50 @@ def test_mc_cooperative_workgroups(oracle, coop):
     exchanging partial dE every step: same decisions and states."""
     assert run_parity(oracle, 4, 300, block=512, replicas=range(0, 32, 8), coop=coop) <= 1
     assert run_parity(oracle, 1, 500, block=256, coop=coop) <= 1
+
+
+@pytest.mark.parametrize("cid", [2, 4])
+def test_mc_windows_off_matches_oracle(oracle, cid):
+    """The one-step-at-a-time kernel (speculative windows switched off)."""
+    assert run_parity(oracle, cid, 600 if cid == 2 else 300, block=1024, replicas=[0, 7, 19] if cid == 2 else [0, 31],
+                      window=False) <= 1
+
+
+def test_mc_windows_with_frequent_conflicts(oracle):
+    """Three small bodies at high temperatures: most speculative windows end
+    early on a body an accepted step of the window already moved, and most
+    corrections between slots are non-zero."""
+    from remcdockingsimulator_amd.engine import Engine
+    from remcdockingsimulator_amd.forcefield import ForceField
+
+    rng = np.random.default_rng(21)
+    ff = ForceField.synthetic(21)
+    sa = synth.build_system([synth.chain_body(5, rng), synth.chain_body(12, rng), synth.crowder_body()], 30.0, rng)
+    lad = [300.0, 600.0, 1200.0]
+    nsteps = 1500
+    for block in (256, 1024):
+        e = Engine(0)
+        e.set_forcefield(ff)
+        e.set_seed(9)
+        sid = e.add_system(sa)
+        e.add_replicas(sid, lad, 1.0, 0.3)
+        e.set_mc_block(block)
+        e.set_mc_coop(1)
+        e.set_recording(nsteps)
+        e.mc_steps(nsteps)
+        e.synchronize()
+        o = oracle.OracleSystem(sa, ff)
+        for r in range(3):
+            say("conflict windows block", block, "replica", r)
+            orep = oracle.OracleReplica(o, lad[r], 1.0, 0.3, r)
+            orec = orep.mc(seed=9, nsteps=nsteps, mode=0)
+            n_eq, div = compare_decisions(e.records(r), orec, lad[r])
+            if not div:
+                st = e.state(r)
+                assert np.array_equal(st["pos"], orep.pos) and np.array_equal(st["quat"], orep.quat)
+                assert abs(e.energies()[r] - orep.energy) < 1e-4 * kT(lad[r])
+        assert np.sum(e.counters()[:, :2]) > nsteps  # plenty of accepted moves
+        e.close()
 
 
 @pytest.mark.parametrize("lds_cache", [False, True])
