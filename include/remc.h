@@ -188,6 +188,24 @@ REMC_API int remc_set_ladder(remc_ctx *ctx, int sys_id, const int *slot_replica)
 REMC_API int remc_get_exchange_state(remc_ctx *ctx, int sys_id, uint64_t *round, int *offset, uint64_t *swaps);
 REMC_API int remc_set_exchange_state(remc_ctx *ctx, int sys_id, uint64_t round, int offset, uint64_t swaps);
 
+/* ------------------------------------------------------------ analysis --- */
+/* Distance RMS of every replica of system sys_id against a reference
+ * conformation of its beads (ref_x/y/z, n_beads each): the mean over pairs of
+ * beads in different non-crowder bodies of | |r_i - r_j| - |r_i - r_j|_ref |
+ * (plain Euclidean distances in FP32, sums in FP64), as DRMS over the
+ * non-crowder molecules ("post processor/pdbgensrc/drms.cpp:7-58").
+ * out[k] for the system's k-th replica. */
+REMC_API int remc_drms(remc_ctx *ctx, int sys_id, const float *ref_x, const float *ref_y, const float *ref_z,
+                       double *out);
+/* Histogram of body-centre distances (minimum image, FP64) over every
+ * replica of system sys_id, by body type ("post processor/gr.py:93-140"):
+ * body_type[m] in [0, n_types), or -1 to leave body m out; bins of width
+ * (hi - lo) / nbin from lo.  counts[(ti * n_types + tj) * nbin + bin] for
+ * ti <= tj: a pair of one type counts from both bodies, a pair of two types
+ * once.  At most 12288 bins in all (n_types^2 x nbin). */
+REMC_API int remc_centre_histogram(remc_ctx *ctx, int sys_id, const int *body_type, int n_types, int nbin, double lo,
+                                   double hi, uint64_t *counts);
+
 /* ------------------------------------------------------------- adjoint --- */
 /* Per-body force and torque about the body centre (6 doubles per body:
  * Fx Fy Fz Tx Ty Tz, kcal/mol/A and kcal/mol/rad, F = -dE/dx) and the

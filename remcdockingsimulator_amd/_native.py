@@ -80,6 +80,9 @@ SIGNATURES = {
     "remc_get_exchange_state": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64),
                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]),
     "remc_set_exchange_state": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]),
+    "remc_drms": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, _dp]),
+    "remc_centre_histogram": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]),
     "remc_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
     "remc_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),

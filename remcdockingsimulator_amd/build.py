@@ -15,9 +15,10 @@ REPO = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libremc.so")
 SIM = os.path.join(HERE, "remc_sim")
+PDBGEN = os.path.join(HERE, "remc_pdbgen")
 
-SOURCES = ["remc_kernels.hip", "remc_api.hip"]
-HOST_SOURCES = ["host/remc_facade.cpp", "host/remc_sim.cpp"]
+SOURCES = ["remc_kernels.hip", "remc_analysis.hip", "remc_api.hip"]
+FACADE_SOURCES = ["host/remc_facade.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -45,16 +46,18 @@ GXX_FLAGS = ["-O2", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unknown-pr
 
 
 def build_sim(verbose=False):
-    """remc_sim: the REMC driver executable, linked to libremc.so beside it."""
+    """remc_sim (the REMC driver) and remc_pdbgen (bound-conformation replay),
+    linked to libremc.so beside them."""
     inc = ["-I", os.path.join(REPO, "include"), "-I", CSRC, "-I", os.path.join(CSRC, "host")]
-    srcs = [os.path.join(CSRC, s) for s in HOST_SOURCES]
-    tmp = SIM + ".tmp"
-    cmd = [os.environ.get("CXX", "g++")] + GXX_FLAGS + inc + srcs + ["-L", HERE, "-lremc",
-                                                                    "-Wl,-rpath,$ORIGIN", "-o", tmp]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    os.replace(tmp, SIM)
+    for exe, main in ((SIM, "host/remc_sim.cpp"), (PDBGEN, "host/remc_pdbgen.cpp")):
+        srcs = [os.path.join(CSRC, s) for s in (FACADE_SOURCES + [main])]
+        tmp = exe + ".tmp"
+        cmd = [os.environ.get("CXX", "g++")] + GXX_FLAGS + inc + srcs + ["-L", HERE, "-lremc",
+                                                                        "-Wl,-rpath,$ORIGIN", "-o", tmp]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        os.replace(tmp, exe)
     return SIM
 
 

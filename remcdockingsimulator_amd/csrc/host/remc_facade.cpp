@@ -1396,3 +1396,67 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
 }
 
 } // namespace remc_facade
+
+namespace remc_facade {
+
+void Replica::writeToTrajectory(FILE *o, int frame, double drms) const
+{
+    // "post processor/postprocess/main.cpp:252-291" record layout
+    std::fprintf(o, "TITLE     frame t=%7.3f\n", (double)(float)frame);
+    std::fprintf(o, "MODEL%9d\n", frame);
+    std::fprintf(o, "REMARK potential,drms: %0.10f %0.10f \n", potential, drms);
+    char chain = 'A';
+    int item = 0;
+    for (const Molecule &m : molecules) {
+        int last = 0;
+        for (const Residue &r : m.Residues) {
+            ++item;
+            std::fprintf(o, "ATOM  %5d %4s%c%3s %c%4d%c  %8.3f%8.3f%8.3f%6.2f%6.2f\n", item, "CA", ' ',
+                         aminoAcids.data[r.aminoAcidIndex].sname.c_str(), chain, r.resSeq, ' ', (double)r.position.x,
+                         (double)r.position.y, (double)r.position.z, 1.0, 1.0);
+            last = r.resSeq;
+        }
+        if (!m.Residues.empty())
+            std::fprintf(o, "TER   %5d      %3s %c%4d \n", item,
+                         aminoAcids.data[m.Residues.back().aminoAcidIndex].sname.c_str(), chain, last);
+        ++chain;
+    }
+    std::fprintf(o, "ENDMDL\n");
+    std::fflush(o);
+}
+
+double DRMS(const Replica &sim, const Replica &exp, bool reverseOrder)
+{
+    auto dist = [](const Vector3f &a, const Vector3f &b) {
+        const double dx = (double)(a.x - b.x), dy = (double)(a.y - b.y), dz = (double)(a.z - b.z);
+        return std::sqrt(dx * dx + dy * dy + dz * dz);
+    };
+    const size_t M = sim.molecules.size();
+    double drms = 0.0, n = 0.0;
+    for (size_t a = 0; a < M; ++a)
+        for (size_t b = a + 1; b < M; ++b)
+            for (size_t i = 0; i < sim.molecules[a].Residues.size(); ++i)
+                for (size_t j = 0; j < sim.molecules[b].Residues.size(); ++j) {
+                    n += 1.0;
+                    drms += std::fabs(dist(sim.molecules[a].Residues[i].position, sim.molecules[b].Residues[j].position) -
+                                      dist(exp.molecules[a].Residues[i].position, exp.molecules[b].Residues[j].position));
+                }
+    drms /= n;
+    bool same_sizes = true; // the swapped comparison pairs residue i of one molecule with residue i of the other
+    for (size_t a = 0; a + 1 < M; ++a)
+        same_sizes = same_sizes && sim.molecules[a].Residues.size() == sim.molecules[a + 1].Residues.size();
+    if (reverseOrder && same_sizes) {
+        double d2 = 0.0;
+        for (size_t a = 0; a < M; ++a)
+            for (size_t b = a + 1; b < M; ++b)
+                for (size_t i = 0; i < sim.molecules[a].Residues.size(); ++i)
+                    for (size_t j = 0; j < sim.molecules[b].Residues.size(); ++j)
+                        d2 += std::fabs(
+                            dist(sim.molecules[a].Residues[i].position, sim.molecules[b].Residues[j].position) -
+                            dist(exp.molecules[b].Residues[i].position, exp.molecules[a].Residues[j].position));
+        drms = std::min(drms, d2 / n);
+    }
+    return drms;
+}
+
+} // namespace remc_facade

@@ -185,6 +185,8 @@ class Replica {
     // Full energy on the GPU (Replica::E / EonDevice); a single-replica context.
     double E(int device = 0);
     void saveAsSinglePDB(const char *filename) const; // Replica.cpp:495-530
+    // one MODEL of a multi-model trajectory ("post processor/postprocess/main.cpp:252-291")
+    void writeToTrajectory(FILE *file, int frame, double drms) const;
 
     // system packing for the C ABI
     struct Packed {
@@ -196,6 +198,12 @@ class Replica {
     };
     Packed pack() const;
 };
+
+// Distance RMS between two conformations of the same molecules: the mean over
+// inter-molecule residue pairs of | |r_i - r_j|_a - |r_i - r_j|_b |; with
+// reverseOrder, the smaller of that and the value with b's two molecules
+// swapped (two equal-sized molecules) ("post processor/pdbgensrc/drms.cpp:7-58").
+double DRMS(const Replica &sim, const Replica &exp, bool reverseOrder);
 
 // force field of an AminoAcids table with the reference constants (definitions.h:18-35)
 remc_forcefield makeForceField(const AminoAcids &aas);

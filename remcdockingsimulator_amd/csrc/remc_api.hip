@@ -114,6 +114,8 @@ struct remc_ctx {
     unsigned long long *d_slots = nullptr;
     int *d_err = nullptr;
     double *d_adj = nullptr; // adjoint scratch: bead gradients, tile partials, body forces, parameter gradient
+    void *d_ana = nullptr;   // analysis scratch (reference conformation, partials, histogram)
+    size_t ana_bytes = 0;
     DevFF devff{};
 
     // MC configuration: workgroup size and workgroups per replica (0 = autotune)
@@ -354,7 +356,7 @@ int finalize(remc_ctx *c)
     HIP_TRY(dalloc(&c->d_cy, cy.size() * REMC_MAX_COOP));
     HIP_TRY(dalloc(&c->d_cz, cz.size() * REMC_MAX_COOP));
     HIP_TRY(dalloc(&c->d_quat, quat.size() * REMC_MAX_COOP));
-    HIP_TRY(dalloc(&c->d_slots, (size_t)R * 2 * REMC_MAX_COOP));
+    HIP_TRY(dalloc(&c->d_slots, (size_t)R * REMC_SLOTS_PER_REP));
     HIP_TRY(dalloc(&c->d_err, (size_t)1));
     HIP_TRY(dalloc(&c->d_E, (size_t)R));
     HIP_TRY(dalloc(&c->d_T, (size_t)R));
@@ -402,7 +404,7 @@ int finalize(remc_ctx *c)
     HIP_TRY(upload(c->d_exswaps, zeros_s, s));
     HIP_TRY(upload(c->d_exoff, zeros_si, s));
     {
-        std::vector<unsigned long long> empty((size_t)R * 2 * REMC_MAX_COOP, ~0ull);
+        std::vector<unsigned long long> empty((size_t)R * REMC_SLOTS_PER_REP, ~0ull);
         HIP_TRY(upload(c->d_slots, empty, s));
         int zero = 0;
         HIP_TRY(hipMemcpyAsync(c->d_err, &zero, sizeof(int), hipMemcpyHostToDevice, s));
@@ -684,7 +686,8 @@ int remc_ctx_destroy(remc_ctx *c)
                     c->d_sys, c->d_rep, c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz,
                     c->d_cx, c->d_cy, c->d_cz, c->d_quat, c->d_E, c->d_T, c->d_step, c->d_cnt,
                     c->d_slot, c->d_exoff, c->d_ladder, c->d_exround, c->d_exswaps, c->d_full_partials,
-                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err, c->d_adj};
+                    c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err, c->d_adj,
+                    c->d_ana};
     for (void *p : ptrs)
         if (p) hipFree(p);
     for (auto e : c->free_events) hipEventDestroy(e);
@@ -1308,6 +1311,83 @@ int remc_set_exchange_state(remc_ctx *c, int sys_id, uint64_t round, int offset,
     HIP_TRY(hipMemcpyAsync(c->d_exround + sys_id, &rd, 8, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_exoff + sys_id, &offset, 4, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpyAsync(c->d_exswaps + sys_id, &sw, 8, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+// Grows the analysis scratch to at least `bytes`.
+static int ana_reserve(remc_ctx *c, size_t bytes)
+{
+    if (c->ana_bytes >= bytes) return REMC_OK;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->d_ana) hipFree(c->d_ana);
+    c->d_ana = nullptr;
+    c->ana_bytes = 0;
+    HIP_TRY(hipMalloc(&c->d_ana, bytes));
+    c->ana_bytes = bytes;
+    return REMC_OK;
+}
+
+int remc_drms(remc_ctx *c, int sys_id, const float *ref_x, const float *ref_y, const float *ref_z, double *out)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "bad system %d", sys_id);
+    if (!ref_x || !ref_y || !ref_z || !out) return fail(REMC_ERR_ARG, "null argument");
+    const HostSys &s = c->systems[sys_id];
+    const int N = s.n_beads, R = (int)s.replicas.size();
+    for (int b = 0; b < N; ++b)
+        if (!std::isfinite(ref_x[b]) || !std::isfinite(ref_y[b]) || !std::isfinite(ref_z[b]))
+            return fail(REMC_ERR_ARG, "non-finite reference coordinate");
+    double pairs = 0.0, beads_before = 0.0; // pairs of beads of different non-crowder bodies
+    for (int m = 0; m < s.n_bodies; ++m) {
+        if (s.crowder[m]) continue;
+        const double nm = s.body_start[m + 1] - s.body_start[m];
+        pairs += nm * beads_before;
+        beads_before += nm;
+    }
+    if (pairs <= 0.0) return fail(REMC_ERR_ARG, "system %d has fewer than two non-crowder bodies", sys_id);
+    const int tiles = (N + 255) / 256;
+    const size_t fbytes = ((3 * (size_t)N * sizeof(float) + 255) / 256) * 256;
+    int rc = ana_reserve(c, fbytes + sizeof(double) * ((size_t)R * tiles + R));
+    if (rc != REMC_OK) return rc;
+    float *rx = (float *)c->d_ana, *ry = rx + N, *rz = ry + N;
+    double *part = (double *)((char *)c->d_ana + fbytes), *dout = part + (size_t)R * tiles;
+    HIP_TRY(hipMemcpyAsync(rx, ref_x, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(ry, ref_y, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(rz, ref_z, sizeof(float) * N, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(launch_drms(c->params(), sys_id, N, R, rx, ry, rz, part, 1.0 / pairs, dout, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_centre_histogram(remc_ctx *c, int sys_id, const int *body_type, int n_types, int nbin, double lo, double hi,
+                          uint64_t *counts)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "bad system %d", sys_id);
+    if (!body_type || !counts) return fail(REMC_ERR_ARG, "null argument");
+    if (n_types < 1 || nbin < 1 || !(hi > lo) || !std::isfinite(lo) || !std::isfinite(hi))
+        return fail(REMC_ERR_ARG, "need n_types >= 1, nbin >= 1 and lo < hi");
+    const size_t nh = (size_t)n_types * n_types * nbin;
+    if (nh * sizeof(unsigned int) > 48 * 1024)
+        return fail(REMC_ERR_LIMIT, "histogram of %zu bins exceeds the 12288-bin workgroup limit", nh);
+    const HostSys &s = c->systems[sys_id];
+    const int M = s.n_bodies, R = (int)s.replicas.size();
+    for (int m = 0; m < M; ++m)
+        if (body_type[m] < -1 || body_type[m] >= n_types) return fail(REMC_ERR_ARG, "body %d has type %d", m, body_type[m]);
+    const size_t tbytes = ((sizeof(int) * (size_t)M + 255) / 256) * 256;
+    int rc = ana_reserve(c, tbytes + sizeof(unsigned long long) * nh);
+    if (rc != REMC_OK) return rc;
+    int *dtype = (int *)c->d_ana;
+    unsigned long long *dcounts = (unsigned long long *)((char *)c->d_ana + tbytes);
+    HIP_TRY(hipMemcpyAsync(dtype, body_type, sizeof(int) * M, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(dcounts, 0, sizeof(unsigned long long) * nh, c->stream));
+    HIP_TRY(launch_centre_hist(c->params(), sys_id, M, R, dtype, n_types, nbin, lo, (hi - lo) / nbin, dcounts,
+                               c->stream));
+    HIP_TRY(hipMemcpyAsync(counts, dcounts, sizeof(unsigned long long) * nh, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
 }

@@ -78,7 +78,7 @@ struct KParams {
     int coop_min_pairs;     // moves with fewer pair terms are not split across members
     size_t copy_beads;      // stride between state copies (beads)
     size_t copy_bodies;     // stride between state copies (bodies)
-    unsigned long long *slots; // n_rep * 2 * REMC_MAX_COOP
+    unsigned long long *slots; // n_rep * REMC_SLOTS_PER_REP
     int *err;               // set when a member waited too long
     // MC dynamic LDS: moved-body staging size and partner-bead/body caches
     int lds_mb;             // largest body (beads)
@@ -106,11 +106,18 @@ hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int b
 hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
 hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
                           double *bead_grad, double *param_part, double *force_torque, double *dparams, hipStream_t s);
+hipError_t launch_drms(const KParams &p, int sys_id, int n_beads, int n_rep, const float *rx, const float *ry,
+                       const float *rz, double *partials, double inv_pairs, double *out, hipStream_t s);
+hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, int n_types,
+                              int nbin, double lo, double dr, unsigned long long *counts, hipStream_t s);
 size_t adjoint_ff_size();
 void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh);
 
 constexpr int FULL_TILE = 256;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
+constexpr int REMC_SLOT_K = 16;  // partial sums per member and exchange (window slots; 1 for a split step)
+// two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica
+constexpr int REMC_SLOTS_PER_REP = 2 * REMC_SLOT_K * REMC_MAX_COOP;
 
 } // namespace remc

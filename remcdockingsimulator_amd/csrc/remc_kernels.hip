@@ -221,6 +221,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ double s_wbase[SPEC ? NW : 1], s_wcorr[SPEC ? NW * NW : 1], s_wdE[SPEC ? NW : 1];
     __shared__ int s_wdec[SPEC ? NW : 1];
     __shared__ int s_weff;
+    __shared__ double s_E;
+    __shared__ unsigned long long s_cnt[3];
     extern __shared__ float4 smem[];
 
     char *lp = (char *)smem;
@@ -248,11 +250,14 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
     if (tid == 0) s_abort = 0;
 
-    double E = p.E[r];
+    // the tracked energy and the counters live in LDS, updated by thread 0
+    // (fewer loop-carried registers per thread)
+    if (tid == 0) {
+        s_E = p.E[r];
+        s_cnt[0] = s_cnt[1] = s_cnt[2] = 0ull;
+    }
     const double beta = remc_beta(p.T[r]);
     unsigned long long step = p.step[r];
-    unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
-    const unsigned long long rec_start = p.rec ? p.rec_start[r] : 0ull;
 
     // private copy c of this replica's dynamic state (always current in HBM)
     const size_t cb = (size_t)c * p.copy_beads + R.bead_off;
@@ -265,8 +270,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
     const int cjs = (int)(((long long)c * N) / coop), cje = (int)(((long long)(c + 1) * N) / coop);
-    unsigned long long *slots = p.slots + (size_t)r * 2 * REMC_MAX_COOP;
-    unsigned int n_split = 0; // split steps so far in this launch: selects the slot parity
+    unsigned long long *slots = p.slots + (size_t)r * REMC_SLOTS_PER_REP;
+    unsigned int n_x = 0; // partial-sum exchanges so far in this launch: selects the slot bank
 
     // cached partner beads: all of them, or only this member's slice when
     // every move of the launch is split (cache_slice)
@@ -417,7 +422,12 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     s_wcorr[jj * NW + kk] = cv;
                 }
             }
-            // dE0 of slot k = wave k: its moved beads against every other bead
+            // dE0 of slot k = wave k: its moved beads against every other bead;
+            // with cooperating members, against this member's partner slice,
+            // the members' partials then exchanged once for the whole window
+            const unsigned long long wtag = step & 0x7FFFFull;
+            unsigned long long *bank = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
+            const int wjs = coop > 1 ? cjs : 0, wje = coop > 1 ? cje : N;
             if (wave < w) {
                 const int k = wave;
                 const int bk = s_draw[off + k].body;
@@ -427,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 for (int i = 0; i < m1k - m0k; ++i) {
                     const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
                     const float *erow = s_etab + (s_jtc[m0k + i] & 0xff) * REMC_NTYPES;
-                    for (int j = lane; j < N; j += 64) {
+                    for (int j = wjs + lane; j < wje; j += 64) {
                         if (j >= m0k && j < m1k) continue;
                         const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
                         const int tcj = s_jtc[j];
@@ -447,13 +457,64 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     }
                 }
                 const double v = wave_sum_dpp((double)acc - (double)comp);
-                if (lane == 0) s_wbase[k] = v;
+                if (lane == 0) {
+                    if (coop > 1)
+                        __hip_atomic_store(bank + k * REMC_MAX_COOP + c, slot_pack(v, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        s_wbase[k] = v;
+                }
             }
             __syncthreads();
 
             // decisions in step order (wave 0); lane l carries slot l's dE
             if (wave == 0) {
-                double dEl = lane < w ? s_wbase[lane] : 0.0;
+                double dEl = 0.0;
+                if (coop > 1) {
+                    // gather the w x coop partials (item t = k * coop + m, two per lane)
+                    const int nt = w * coop;
+                    const int t0 = lane, t1 = lane + 64;
+                    unsigned long long v0 = 0ull, v1 = 0ull;
+                    bool ok0 = t0 >= nt, ok1 = t1 >= nt;
+                    int spins = 0;
+                    bool failed = false;
+                    while (!__all(ok0 && ok1)) {
+                        if (!ok0) {
+                            v0 = __hip_atomic_load(bank + (t0 / coop) * REMC_MAX_COOP + t0 % coop, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            ok0 = (v0 & SLOT_TAG_MASK) == wtag;
+                        }
+                        if (!ok1) {
+                            v1 = __hip_atomic_load(bank + (t1 / coop) * REMC_MAX_COOP + t1 % coop, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                            ok1 = (v1 & SLOT_TAG_MASK) == wtag;
+                        }
+                        if (++spins >= SLOT_SPIN_LIMIT) {
+                            failed = true;
+                            break;
+                        }
+                        if ((spins & 255) == 0 && __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                            failed = true;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    if (failed) {
+                        if (lane == 0) {
+                            __hip_atomic_fetch_or(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            s_abort = 1;
+                        }
+                    }
+                    // slot k's total, members added in member order: identical in every member
+                    for (int m = 0; m < coop; ++m) {
+                        const int t = lane * coop + m;
+                        const unsigned long long a0 = __shfl(v0, t & 63, 64), a1 = __shfl(v1, t & 63, 64);
+                        dEl += slot_value(t < 64 ? a0 : a1);
+                    }
+                    if (lane >= w) dEl = 0.0;
+                } else {
+                    dEl = lane < w ? s_wbase[lane] : 0.0;
+                }
                 const int bl = lane < w ? s_draw[off + lane].body : -1;
                 unsigned long long accm = 0ull;
                 int weff = w;
@@ -478,6 +539,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (lane == 0) s_weff = weff;
             }
             __syncthreads();
+            if (s_abort) break;
+            if (coop > 1) ++n_x;
             const int weff = s_weff;
 
             // commit the accepted moves (their bodies are distinct)
@@ -535,16 +598,17 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     }
                 }
             }
-            for (int k = 0; k < weff; ++k) {
-                const int dec = s_wdec[k];
-                if (dec) {
-                    E += s_wdE[k];
-                    if (dec == 1) ++n_acc; else ++n_accb;
-                } else {
-                    ++n_rej;
+            if (tid == 0) {
+                double E = s_E;
+                for (int k = 0; k < weff; ++k) {
+                    const int dec = s_wdec[k];
+                    if (dec) E += s_wdE[k];
+                    ++s_cnt[dec == 1 ? 0 : dec == 2 ? 1 : 2];
                 }
+                s_E = E;
             }
             if (c == 0 && p.rec && tid < weff) {
+                const unsigned long long rec_start = p.rec_start[r];
                 const unsigned long long st = step + tid;
                 const unsigned long long idx = st - rec_start;
                 if (st >= rec_start && idx < (unsigned long long)p.rec_max) {
@@ -676,7 +740,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // slot only after every member has read it (they all wrote
                 // their partial of the split step in between)
                 const unsigned long long tag = step & 0x7FFFFull;
-                unsigned long long *slot = slots + (n_split & 1u) * REMC_MAX_COOP;
+                unsigned long long *slot = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
                 if (lane == 0) __hip_atomic_store(slot + c, slot_pack(part, tag), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 unsigned long long wv = 0ull;
                 bool ok = lane >= coop;
@@ -712,7 +776,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             __syncthreads();
             if (s_abort) break;
             dE = s_dE;
-            ++n_split;
+            ++n_x;
         }
 
         const int decision = remc_metropolis_l(dE, beta, D.nlu);
@@ -764,12 +828,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     }
                 }
             }
-            E += dE;
-            if (decision == 1) ++n_acc; else ++n_accb;
-        } else {
-            ++n_rej;
+        }
+        if (tid == 0) {
+            if (decision) s_E += dE;
+            ++s_cnt[decision == 1 ? 0 : decision == 2 ? 1 : 2];
         }
         if (c == 0 && p.rec && tid == 0) {
+            const unsigned long long rec_start = p.rec_start[r];
             unsigned long long idx = step - rec_start;
             if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
                 remc_step_record rr;
@@ -786,11 +851,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         ++step;
     }
     if (c == 0 && tid == 0 && !s_abort) {
-        p.E[r] = E;
+        p.E[r] = s_E;
         p.step[r] = step;
-        p.cnt[3 * r + 0] += n_acc;
-        p.cnt[3 * r + 1] += n_accb;
-        p.cnt[3 * r + 2] += n_rej;
+        p.cnt[3 * r + 0] += s_cnt[0];
+        p.cnt[3 * r + 1] += s_cnt[1];
+        p.cnt[3 * r + 2] += s_cnt[2];
     }
 }
 
@@ -831,7 +896,7 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
     // the speculative kernel indexes its caches by bead and body: they must hold the whole replica
     if (p.spec && (p.lds_cl <= 0 || p.cache_slice || !p.cache_bodies)) return hipErrorInvalidValue;
     if (coop > 1) {
-        const int n = n_rep * 2 * REMC_MAX_COOP;
+        const int n = n_rep * REMC_SLOTS_PER_REP;
         hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
     }
     const int grid = n_rep * coop;

@@ -243,6 +243,24 @@ class Engine:
         self._chk(self.lib.remc_set_exchange_state(self.ctx, sys_id, int(round_), int(offset), int(swaps)),
                   "remc_set_exchange_state")
 
+    # -------------------------------------------------------- analysis ----
+    def drms(self, sys_id, ref_pos):
+        """DRMS of every replica of a system against ref_pos (n_beads, 3)."""
+        ref = np.asarray(ref_pos, dtype=np.float32)
+        cols = [np.ascontiguousarray(ref[:, k]) for k in range(3)]
+        out = np.zeros(self.replica_sys.count(sys_id), dtype=np.float64)
+        self._chk(self.lib.remc_drms(self.ctx, sys_id, *(nat.fptr(c) for c in cols), nat.dptr(out)), "remc_drms")
+        return out
+
+    def centre_histogram(self, sys_id, body_type, n_types, nbin, lo, hi):
+        """Counts (n_types, n_types, nbin) of body-centre distances over the system's replicas."""
+        bt = np.ascontiguousarray(body_type, dtype=np.int32)
+        out = np.zeros(n_types * n_types * nbin, dtype=np.uint64)
+        self._chk(self.lib.remc_centre_histogram(self.ctx, sys_id, nat.iptr(bt), int(n_types), int(nbin), float(lo),
+                                                 float(hi), out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))),
+                  "remc_centre_histogram")
+        return out.reshape(n_types, n_types, nbin)
+
     # ---------------------------------------------------------- timing ----
     MC, FULL, PLAIN, EXCHANGE, ADJOINT = range(5)
 
