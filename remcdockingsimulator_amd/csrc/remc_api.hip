@@ -497,7 +497,7 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
     const int R = (int)c->reps.size();
     const long long wgs = (long long)R * coop;
     const int per_cu = (int)std::max<long long>(1, (wgs + c->n_cu - 1) / c->n_cu);
-    const int static_lds = 24 * 1024; // etab, draws, window staging, reduction scratch and slack
+    const int static_lds = 28 * 1024; // etab, draws, window staging, reduction scratch and slack
     long long budget = std::min<long long>(c->lds_per_block, c->lds_per_cu / per_cu) - static_lds;
     p.lds_mb = std::max(1, c->max_body_beads);
     budget -= 36LL * p.lds_mb;

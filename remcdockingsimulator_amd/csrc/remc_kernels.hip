@@ -199,12 +199,12 @@ size_t mc_dynamic_lds(const KParams &p)
 // j < k of the window differs from dE0_k only by the pair terms between the
 // two moved bodies,
 //   corr(j,k) = sum_{i in k, l in j} [e(n_i,n_l) - e(o_i,n_l)] - [e(n_i,o_l) - e(o_i,o_l)],
-// which are formed for every pair of slots in parallel; one wave then decides
-// the steps in order, adding corr(j,k) to every later slot when step j is
-// accepted.  A step that moves a body an accepted step of the window already
-// moved ends the window (its proposal depends on that move); it starts the
-// next one.  Decisions, energies and states are those of taking the steps one
-// at a time, up to the FP32 rounding of the reassociated pair sums.
+// which are formed for every pair of slots in parallel.  Every slot is decided
+// on its dE0; the window ends before the first slot k that an accepted earlier
+// slot j interacts with (corr(j,k) != 0, or the same body, whose proposal
+// depends on j's move), so every kept decision used its exact dE; slot k starts
+// the next window.  Decisions, energies and states are those of taking the
+// steps one at a time, up to the FP32 rounding of the reassociated pair sums.
 template <int BLOCK, bool CACHE, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
@@ -217,8 +217,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ int s_abort;
     __shared__ StepDraw s_draw[DRAW_BATCH];
     __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
-    __shared__ float4 s_wold[WS], s_wnew[WS];
-    __shared__ double s_wbase[SPEC ? NW : 1], s_wcorr[SPEC ? NW * NW : 1], s_wdE[SPEC ? NW : 1];
+    __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old/new position, new relative position
+    __shared__ float4 s_wcen[SPEC ? NW : 1];                // new centre of a translated slot
+    __shared__ double s_wq[SPEC ? 4 * NW : 1];              // new rotation of a rotated slot
+    __shared__ double s_wbase[SPEC ? NW : 1], s_wdE[SPEC ? NW : 1];
+    __shared__ unsigned int s_wint[SPEC ? NW : 1]; // bit j of slot k: slot j's move changes slot k's dE
     __shared__ int s_wdec[SPEC ? NW : 1];
     __shared__ int s_weff;
     __shared__ double s_E;
@@ -359,6 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
             // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
+            if (tid < NW) s_wint[tid] = 0u;
             if (tid < NW * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < w) {
@@ -376,52 +380,22 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         else remc_wrap_centre(ncx, ncy, ncz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
                         s_wold[tid] = make_float4(s_jx[b], s_jy[b], s_jz[b], s_jq[b] * ff.c_dh);
                         s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, s_js[b]);
+                        s_wrel[tid] = make_float4(nrx, nry, nrz, 0.0f);
+                        if (i == 0) {
+                            if (Dk.rotate) {
+                                double qo[4], qn[4];
+                                for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * bk + t] : gq[4 * bk + t];
+                                remc_qmul(Dk.q, qo, qn);
+                                for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
+                            } else {
+                                s_wcen[k] = make_float4(ncx, ncy, ncz, 0.0f);
+                            }
+                        }
                     }
                 }
             }
             __syncthreads();
 
-            // pair corrections between the moved bodies of every two slots j < k
-            if (tid < NW * NW) {
-                const int jj = tid / NW, kk = tid % NW;
-                if (jj < kk && kk < w) {
-                    const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
-                    double cv = 0.0;
-                    if (bj != bk) {
-                        const int m0j = s_bs[bj], nmj = s_be[bj] - m0j;
-                        const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
-                        const bool crow = ((s_jtc[m0j] | s_jtc[m0k]) >> 8) != 0;
-                        float acc = 0.0f, comp = 0.0f;
-                        for (int i = 0; i < nmk; ++i) {
-                            const float4 o = s_wold[kk * SPEC_MAX + i], n = s_wnew[kk * SPEC_MAX + i];
-                            const int ti = s_jtc[m0k + i] & 0xff;
-                            for (int l = 0; l < nmj; ++l) {
-                                const float4 O = s_wold[jj * SPEC_MAX + l], Nn = s_wnew[jj * SPEC_MAX + l];
-                                float e_nn, e_on, e_no, e_oo;
-                                if (crow) {
-                                    e_nn = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, n, S), ff);
-                                    e_on = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, o, S), ff);
-                                    e_no = pair_crowder(dist2(O.x, O.y, O.z, n, S), ff);
-                                    e_oo = pair_crowder(dist2(O.x, O.y, O.z, o, S), ff);
-                                } else {
-                                    const int bl = m0j + l;
-                                    const float eij = s_etab[ti * REMC_NTYPES + (s_jtc[bl] & 0xff)];
-                                    const float sij = 0.5f * (n.w + s_js[bl]);
-                                    const float sij2 = sij * sij;
-                                    const float qq = o.w * s_jq[bl];
-                                    e_nn = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, n, S), eij, sij2, qq, ff);
-                                    e_on = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, o, S), eij, sij2, qq, ff);
-                                    e_no = pair_lj_dh(dist2(O.x, O.y, O.z, n, S), eij, sij2, qq, ff);
-                                    e_oo = pair_lj_dh(dist2(O.x, O.y, O.z, o, S), eij, sij2, qq, ff);
-                                }
-                                kahan_add(acc, comp, (e_nn - e_on) - (e_no - e_oo));
-                            }
-                        }
-                        cv = (double)acc - (double)comp;
-                    }
-                    s_wcorr[jj * NW + kk] = cv;
-                }
-            }
             // dE0 of slot k = wave k: its moved beads against every other bead;
             // with cooperating members, against this member's partner slice,
             // the members' partials then exchanged once for the whole window
@@ -463,6 +437,49 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                                            __HIP_MEMORY_SCOPE_AGENT);
                     else
                         s_wbase[k] = v;
+                }
+            }
+            // pair corrections between the moved bodies of every two slots j < k
+            if (tid < NW * NW) {
+                const int jj = tid / NW, kk = tid % NW;
+                if (jj < kk && kk < w) {
+                    const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
+                    float cv = 0.0f;
+                    bool touches = bj == bk; // the same body: slot kk's proposal depends on slot jj's move
+                    if (bj != bk) {
+                        const int m0j = s_bs[bj], nmj = s_be[bj] - m0j;
+                        const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                        const bool crow = ((s_jtc[m0j] | s_jtc[m0k]) >> 8) != 0;
+                        float acc = 0.0f, comp = 0.0f;
+                        for (int i = 0; i < nmk; ++i) {
+                            const float4 o = s_wold[kk * SPEC_MAX + i], n = s_wnew[kk * SPEC_MAX + i];
+                            const int ti = s_jtc[m0k + i] & 0xff;
+                            for (int l = 0; l < nmj; ++l) {
+                                const float4 O = s_wold[jj * SPEC_MAX + l], Nn = s_wnew[jj * SPEC_MAX + l];
+                                float e_nn, e_on, e_no, e_oo;
+                                if (crow) {
+                                    e_nn = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, n, S), ff);
+                                    e_on = pair_crowder(dist2(Nn.x, Nn.y, Nn.z, o, S), ff);
+                                    e_no = pair_crowder(dist2(O.x, O.y, O.z, n, S), ff);
+                                    e_oo = pair_crowder(dist2(O.x, O.y, O.z, o, S), ff);
+                                } else {
+                                    const int bl = m0j + l;
+                                    const float eij = s_etab[ti * REMC_NTYPES + (s_jtc[bl] & 0xff)];
+                                    const float sij = 0.5f * (n.w + s_js[bl]);
+                                    const float sij2 = sij * sij;
+                                    const float qq = o.w * s_jq[bl];
+                                    e_nn = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, n, S), eij, sij2, qq, ff);
+                                    e_on = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, o, S), eij, sij2, qq, ff);
+                                    e_no = pair_lj_dh(dist2(O.x, O.y, O.z, n, S), eij, sij2, qq, ff);
+                                    e_oo = pair_lj_dh(dist2(O.x, O.y, O.z, o, S), eij, sij2, qq, ff);
+                                }
+                                kahan_add(acc, comp, (e_nn - e_on) - (e_no - e_oo));
+                            }
+                        }
+                        cv = acc - comp;
+                        touches = cv != 0.0f;
+                    }
+                    if (touches) atomicOr(&s_wint[kk], 1u << jj);
                 }
             }
             __syncthreads();
@@ -515,26 +532,38 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 } else {
                     dEl = lane < w ? s_wbase[lane] : 0.0;
                 }
-                const int bl = lane < w ? s_draw[off + lane].body : -1;
-                unsigned long long accm = 0ull;
-                int weff = w;
-                for (int k = 0; k < w; ++k) {
-                    const int bk = __shfl(bl, k, 64);
-                    const bool moved = lane < k && ((accm >> lane) & 1ull) && bl == bk;
-                    if (__ballot(moved) != 0ull) {
-                        weff = k;
-                        break;
+                // Every slot is decided at once on its dE0.  That is step k's
+                // exact dE unless an accepted earlier slot j of the window moved a
+                // body that interacts with slot k's (corr(j,k) != 0) or moved the
+                // same body.  The window ends before the first such slot; the
+                // decisions before it are exact, and it starts the next window,
+                // staged from the updated conformation.
+                const double nll = lane < w ? s_draw[off + lane].nlu : 0.0;
+                const int dl = lane < w ? remc_metropolis_l(dEl, beta, nll) : 0;
+                const unsigned long long acc0 = __ballot(dl != 0);
+                const unsigned int il = lane < w ? s_wint[lane] : 0u;
+                const bool inter = (il & (unsigned int)acc0) != 0u;
+                const unsigned long long stop = __ballot(inter) | (w < 64 ? (1ull << w) : 0ull);
+                const int weff = (int)__builtin_ctzll(stop);
+                if (lane < weff) {
+                    s_wdec[lane] = dl;
+                    s_wdE[lane] = dEl;
+                }
+                const unsigned long long accm = weff < 64 ? acc0 & ((1ull << weff) - 1ull) : acc0;
+                const unsigned long long boltzm = __ballot(dl == 2) & accm;
+                if (lane == 0) {
+                    // the tracked energy gains the accepted dE in step order, as one step at a time does
+                    double Ew = s_E;
+                    for (unsigned long long m = accm; m; m &= m - 1ull) {
+                        const int k = (int)__builtin_ctzll(m);
+                        Ew += __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(dEl), k),
+                                               __builtin_amdgcn_readlane(__double2loint(dEl), k));
                     }
-                    const double dEk = __shfl(dEl, k, 64);
-                    const int dec = remc_metropolis_l(dEk, beta, s_draw[off + k].nlu);
-                    if (lane == 0) {
-                        s_wdec[k] = dec;
-                        s_wdE[k] = dEk;
-                    }
-                    if (dec) {
-                        accm |= 1ull << k;
-                        if (lane > k && lane < w) dEl += s_wcorr[k * NW + lane];
-                    }
+                    s_E = Ew;
+                    const int nacc = __popcll(accm), nbz = __popcll(boltzm);
+                    s_cnt[0] += (unsigned long long)(nacc - nbz);
+                    s_cnt[1] += (unsigned long long)nbz;
+                    s_cnt[2] += (unsigned long long)(weff - nacc);
                 }
                 if (lane == 0) s_weff = weff;
             }
@@ -543,7 +572,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (coop > 1) ++n_x;
             const int weff = s_weff;
 
-            // commit the accepted moves (their bodies are distinct)
+            // commit the accepted moves (their bodies are distinct): copies of
+            // what staging formed
             if (tid < NW * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < weff && s_wdec[k]) {
@@ -560,52 +590,35 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         s_jy[b] = n.y;
                         s_jz[b] = n.z;
                         if (Dk.rotate) {
-                            const float rx = cache_rel ? s_rx[b] : grx[b];
-                            const float ry = cache_rel ? s_ry[b] : gry[b];
-                            const float rz = cache_rel ? s_rz[b] : grz[b];
-                            float nrx, nry, nrz;
-                            remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
-                            grx[b] = nrx;
-                            gry[b] = nry;
-                            grz[b] = nrz;
+                            const float4 nr = s_wrel[tid];
+                            grx[b] = nr.x;
+                            gry[b] = nr.y;
+                            grz[b] = nr.z;
                             if (cache_rel) {
-                                s_rx[b] = nrx;
-                                s_ry[b] = nry;
-                                s_rz[b] = nrz;
+                                s_rx[b] = nr.x;
+                                s_ry[b] = nr.y;
+                                s_rz[b] = nr.z;
                             }
                         }
                         if (i == 0) {
                             if (Dk.rotate) {
-                                double qo[4], qn[4];
-                                for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * bk + t] : gq[4 * bk + t];
-                                remc_qmul(Dk.q, qo, qn);
                                 for (int t = 0; t < 4; ++t) {
-                                    gq[4 * bk + t] = qn[t];
-                                    if (cache_quat) s_q[4 * bk + t] = qn[t];
+                                    const double qn = s_wq[4 * k + t];
+                                    gq[4 * bk + t] = qn;
+                                    if (cache_quat) s_q[4 * bk + t] = qn;
                                 }
                             } else {
-                                float ncx, ncy, ncz;
-                                remc_wrap_centre(s_cx[bk], s_cy[bk], s_cz[bk], Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx,
-                                                 &ncy, &ncz);
-                                gcx[bk] = ncx;
-                                gcy[bk] = ncy;
-                                gcz[bk] = ncz;
-                                s_cx[bk] = ncx;
-                                s_cy[bk] = ncy;
-                                s_cz[bk] = ncz;
+                                const float4 nc = s_wcen[k];
+                                gcx[bk] = nc.x;
+                                gcy[bk] = nc.y;
+                                gcz[bk] = nc.z;
+                                s_cx[bk] = nc.x;
+                                s_cy[bk] = nc.y;
+                                s_cz[bk] = nc.z;
                             }
                         }
                     }
                 }
-            }
-            if (tid == 0) {
-                double E = s_E;
-                for (int k = 0; k < weff; ++k) {
-                    const int dec = s_wdec[k];
-                    if (dec) E += s_wdE[k];
-                    ++s_cnt[dec == 1 ? 0 : dec == 2 ? 1 : 2];
-                }
-                s_E = E;
             }
             if (c == 0 && p.rec && tid < weff) {
                 const unsigned long long rec_start = p.rec_start[r];
