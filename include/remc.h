@@ -146,6 +146,19 @@ REMC_API int remc_mc_steps(remc_ctx *ctx, int nsteps);
  * Metropolis, copy-back on reject (Replica.cpp:386-474).  Uses the same RNG
  * stream and move arithmetic as remc_mc_steps. */
 REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int block);
+/* nsteps force-biased (smart) MC steps on every replica, one workgroup per
+ * replica (Rossky, Doll & Friedman, J. Chem. Phys. 69, 4628 (1978)): the
+ * chosen body's translation is drawn as d = beta A F + sqrt(2A) n, or its
+ * rotation vector as beta A tau + sqrt(2A) n, from the force F on the body or
+ * the torque tau about its centre (n standard normal; A = t^2/6 for
+ * translations, r^2/6 for rotations, t and r the replica's translate and
+ * rotate steps), and accepted with the proposal ratio included, so the chain
+ * keeps exp(-beta E).  No counterpart in the reference: it replaces the
+ * uniform proposal of Replica::MCSearch (Replica.cpp:386-420) with one driven
+ * by the gradient that remc_adjoint returns.  Shares the replicas, energies,
+ * step and decision counters with remc_mc_steps.  block: workgroup size (256,
+ * 512 or 1024; 0 = 256). */
+REMC_API int remc_mc_steps_fb(remc_ctx *ctx, int nsteps, int block);
 /* One replica-exchange round over every system's temperature ladder, pairs
  * (k,k+1) for k = offset, offset+2, ...; the offset alternates per call
  * starting at 0 (main.cpp:1427-1453).  *n_swaps (may be NULL) receives the

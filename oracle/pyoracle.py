@@ -91,6 +91,14 @@ def lib():
                                      ctypes.c_int, ctypes.c_int, _dp]
         L.orc_metropolis.restype = ctypes.c_int
         L.orc_metropolis.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_mc_fb_steps.restype = None
+        L.orc_mc_fb_steps.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield),
+                                      ctypes.POINTER(orc_replica), ctypes.POINTER(ctypes.c_uint32), ctypes.c_int,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                      ctypes.POINTER(orc_step_record)]
+        L.orc_body_force.restype = None
+        L.orc_body_force.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield), _fp, _fp, _fp,
+                                     ctypes.c_int, _fp, _fp, _fp, _fp, _fp, _fp, _dp]
         L.orc_mc_steps.restype = None
         L.orc_mc_steps.argtypes = [ctypes.POINTER(orc_system), ctypes.POINTER(orc_forcefield),
                                    ctypes.POINTER(orc_replica), _u32p, ctypes.c_int, ctypes.c_int,
@@ -258,6 +266,16 @@ class OracleReplica:
         lib().orc_mc_steps(ctypes.byref(self.osys.s), ctypes.byref(self.osys.ff), ctypes.byref(self.r),
                            key_words(seed), nsteps, mode,
                            rec.ctypes.data_as(ctypes.POINTER(orc_step_record)) if record else None)
+        self.sync_out()
+        return rec
+
+    def mc_fb(self, seed, nsteps, a_t, a_r, record=True, hastings=True):
+        """Force-biased MC steps (orc_mc_fb_steps) with mobilities a_t (A^2), a_r (rad^2);
+        hastings=False drops the proposal ratio (a planted mistake for tests)."""
+        rec = np.zeros(nsteps, dtype=RECORD_DTYPE) if record else None
+        lib().orc_mc_fb_steps(ctypes.byref(self.osys.s), ctypes.byref(self.osys.ff), ctypes.byref(self.r),
+                              key_words(seed), nsteps, float(a_t), float(a_r), 1 if hastings else 0,
+                              rec.ctypes.data_as(ctypes.POINTER(orc_step_record)) if record else None)
         self.sync_out()
         return rec
 

@@ -471,7 +471,8 @@ int orc_metropolis(double delta, double temperature, double u, double *margin)
         return 1;
     }
     double p = orc_exp(-delta * beta);
-    if (margin) *margin = fabs(log(u) + delta * beta);
+    /* two thresholds: dE = 0 (accept kind 1 or 2) and -ln u (accept or reject) */
+    if (margin) *margin = fmin(delta * beta, fabs(log(u) + delta * beta));
     return u < p ? 2 : 0;
 }
 
@@ -612,4 +613,153 @@ double orc_mc_threaded(const orc_system *s, const orc_forcefield *ff, orc_replic
     free(th);
     free(args);
     return (double)(t1.tv_sec - t0.tv_sec) + 1e-9 * (double)(t1.tv_nsec - t0.tv_nsec);
+}
+
+/* ------------------------------------------------- force-biased MC ---- */
+#define ORC_TAG_FB 0x46420000u
+
+/* Force (3) and torque about the centre (3) on body m whose beads sit at
+ * (bx,by,bz) with relative positions (brx,bry,brz), from every bead of the
+ * other bodies at (x,y,z).  Pair derivatives as orc_gradient. */
+void orc_body_force(const orc_system *s, const orc_forcefield *ff, const float *x, const float *y, const float *z,
+                    int m, const float *bx, const float *by, const float *bz, const float *brx, const float *bry,
+                    const float *brz, double ft[6])
+{
+    int m0 = s->body_start[m], m1 = s->body_start[m + 1];
+    for (int k = 0; k < 6; ++k) ft[k] = 0.0;
+    for (int i = m0; i < m1; ++i) {
+        int k = i - m0;
+        double f[3] = {0.0, 0.0, 0.0};
+        for (int b = 0; b < s->n_bodies; ++b) {
+            if (b == m) continue;
+            for (int j = s->body_start[b]; j < s->body_start[b + 1]; ++j) {
+                double dx = orc_min_image((double)bx[k] - (double)x[j], s->box);
+                double dy = orc_min_image((double)by[k] - (double)y[j], s->box);
+                double dz = orc_min_image((double)bz[k] - (double)z[j], s->box);
+                double d = sqrt(dx * dx + dy * dy + dz * dz);
+                double r = d + ff->eps_r;
+                double dedr, de_deij, de_dxi;
+                orc_pair_derivs(ff, r, s->type[i], s->charge[i], s->sigma[i], s->crowder[m], s->type[j],
+                                s->charge[j], s->sigma[j], s->crowder[b], &dedr, &de_deij, &de_dxi);
+                double g = d > 0.0 ? dedr / d : 0.0;
+                f[0] -= g * dx; f[1] -= g * dy; f[2] -= g * dz;
+            }
+        }
+        ft[0] += f[0]; ft[1] += f[1]; ft[2] += f[2];
+        double ax = brx[k], ay = bry[k], az = brz[k];
+        ft[3] += ay * f[2] - az * f[1];
+        ft[4] += az * f[0] - ax * f[2];
+        ft[5] += ax * f[1] - ay * f[0];
+    }
+}
+
+void orc_mc_fb_steps(const orc_system *s, const orc_forcefield *ff, orc_replica *r, const uint32_t key[2],
+                     int nsteps, double a_t, double a_r, int hastings, orc_step_record *rec)
+{
+    int maxn = 1;
+    for (int b = 0; b < s->n_bodies; ++b) {
+        int n = s->body_start[b + 1] - s->body_start[b];
+        if (n > maxn) maxn = n;
+    }
+    float *buf = (float *)malloc(sizeof(float) * 6 * (size_t)maxn);
+    float *nx = buf, *ny = buf + maxn, *nz = buf + 2 * maxn;
+    float *nrx = buf + 3 * maxn, *nry = buf + 4 * maxn, *nrz = buf + 5 * maxn;
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int st = 0; st < nsteps; ++st) {
+        uint32_t ctr[4] = {(uint32_t)r->step, (uint32_t)(r->step >> 32), r->rng_id, ORC_TAG_FB};
+        uint32_t w0[4], w1[4];
+        orc_philox4x32_10(ctr, key, w0);
+        ctr[3] = ORC_TAG_FB + 1u;
+        orc_philox4x32_10(ctr, key, w1);
+        int m = (int)(((uint64_t)w0[0] * (uint64_t)s->n_bodies) >> 32);
+        int rotate = orc_u01(w0[1]) < 0.5;
+        double u1 = orc_u01(w0[2]), u2 = orc_u01(w0[3]), u3 = orc_u01(w1[0]), u4 = orc_u01(w1[1]);
+        double u = orc_u01(w1[2]);
+        /* Box-Muller on (1 - u1, u2) and (1 - u3, u4): three standard normals */
+        double rho1 = sqrt(-2.0 * log(1.0 - u1)), rho2 = sqrt(-2.0 * log(1.0 - u3));
+        double nrm[3] = {rho1 * cos(two_pi * u2), rho1 * sin(two_pi * u2), rho2 * cos(two_pi * u4)};
+        const double beta = 4184.0 / (ORC_R_GAS * r->temperature);
+        int m0 = s->body_start[m], n = s->body_start[m + 1] - m0;
+        double ft_old[6], ft_new[6];
+        orc_body_force(s, ff, r->x, r->y, r->z, m, r->x + m0, r->y + m0, r->z + m0, r->rx + m0, r->ry + m0,
+                       r->rz + m0, ft_old);
+        float cnew[3] = {r->cx[m], r->cy[m], r->cz[m]};
+        double qold[4] = {r->qw[m], r->qx[m], r->qy[m], r->qz[m]}, qnew[4];
+        double disp[3];
+        const double A = rotate ? a_r : a_t;
+        const double *fo = rotate ? ft_old + 3 : ft_old;
+        for (int k = 0; k < 3; ++k) disp[k] = beta * A * fo[k] + sqrt(2.0 * A) * nrm[k];
+        if (rotate) {
+            double ang = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+            double q[4] = {1.0, 0.0, 0.0, 0.0};
+            if (ang > 0.0) {
+                double sh = sin(0.5 * ang), ch = cos(0.5 * ang);
+                q[0] = ch; q[1] = sh * disp[0] / ang; q[2] = sh * disp[1] / ang; q[3] = sh * disp[2] / ang;
+                double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+                q[0] = q[0] / qm; q[1] = q[1] / qm; q[2] = q[2] / qm; q[3] = q[3] / qm;
+            }
+            orc_quat_mul(q, qold, qnew);
+            for (int k = 0; k < n; ++k) {
+                float v[3] = {r->rx[m0 + k], r->ry[m0 + k], r->rz[m0 + k]}, o[3];
+                orc_quat_rotate(q, v, o);
+                nrx[k] = o[0]; nry[k] = o[1]; nrz[k] = o[2];
+                nx[k] = o[0] + cnew[0]; ny[k] = o[1] + cnew[1]; nz[k] = o[2] + cnew[2];
+            }
+        } else {
+            float t[3] = {(float)disp[0], (float)disp[1], (float)disp[2]};
+            for (int k = 0; k < 3; ++k) disp[k] = (double)t[k]; /* the displacement actually made */
+            float L = (float)s->box;
+            float px = cnew[0] + t[0], py = cnew[1] + t[1], pz = cnew[2] + t[2];
+            if (L > 0.0f) {
+                px = orc_fmodf(px + L, L);
+                py = orc_fmodf(py + L, L);
+                pz = orc_fmodf(pz + L, L);
+            }
+            cnew[0] = px; cnew[1] = py; cnew[2] = pz;
+            for (int k = 0; k < n; ++k) {
+                nrx[k] = r->rx[m0 + k]; nry[k] = r->ry[m0 + k]; nrz[k] = r->rz[m0 + k];
+                nx[k] = nrx[k] + px; ny[k] = nry[k] + py; nz[k] = nrz[k] + pz;
+            }
+            memcpy(qnew, qold, sizeof qold);
+        }
+        double delta = orc_delta_energy(s, ff, r->x, r->y, r->z, m, nx, ny, nz);
+        orc_body_force(s, ff, r->x, r->y, r->z, m, nx, ny, nz, nrx, nry, nrz, ft_new);
+        const double *fn = rotate ? ft_new + 3 : ft_new;
+        /* log q(new->old) - log q(old->new): reverse displacement -disp */
+        double fwd = 0.0, rev = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            double a = disp[k] - beta * A * fo[k], b = -disp[k] - beta * A * fn[k];
+            fwd += a * a;
+            rev += b * b;
+        }
+        /* accept if x < -ln u; hastings = 0 drops the proposal ratio (a planted
+         * mistake for the tests: the chain then leaves exp(-beta E)) */
+        double x = beta * delta + (hastings ? (rev - fwd) / (4.0 * A) : 0.0);
+        double Lu = u > 0.0 ? -log(u) : 745.1332191019412;
+        int acc = x < 0.0 ? 1 : (x < Lu ? 2 : 0);
+        if (acc) {
+            memcpy(r->x + m0, nx, sizeof(float) * n);
+            memcpy(r->y + m0, ny, sizeof(float) * n);
+            memcpy(r->z + m0, nz, sizeof(float) * n);
+            memcpy(r->rx + m0, nrx, sizeof(float) * n);
+            memcpy(r->ry + m0, nry, sizeof(float) * n);
+            memcpy(r->rz + m0, nrz, sizeof(float) * n);
+            r->cx[m] = cnew[0]; r->cy[m] = cnew[1]; r->cz[m] = cnew[2];
+            r->qw[m] = qnew[0]; r->qx[m] = qnew[1]; r->qy[m] = qnew[2]; r->qz[m] = qnew[3];
+            r->energy += delta;
+            if (acc == 1) r->n_accept++; else r->n_accept_boltz++;
+        } else {
+            r->n_reject++;
+        }
+        if (rec) {
+            rec[st].body = m;
+            rec[st].move = rotate;
+            rec[st].accepted = acc;
+            rec[st].pad = 0;
+            rec[st].delta = delta;
+            rec[st].margin = x < 0.0 ? -x : fmin(x, fabs(x - Lu));
+        }
+        r->step++;
+    }
+    free(buf);
 }

@@ -93,8 +93,9 @@ typedef struct orc_step_record {
     int32_t accepted;    /* 0 reject, 1 accept (dE<0), 2 accept (Boltzmann) */
     int32_t pad;
     double delta;        /* dE, kcal/mol */
-    double margin;       /* distance of the decision from its threshold: |dE| if dE<0,
-                            else log(u) + beta*dE (accept if < 0)             */
+    double margin;       /* distance of the decision from its nearest threshold, in
+                            kT: beta|dE| if dE < 0, else the smaller of beta dE
+                            and |ln u + beta dE| (accept if < 0)              */
 } orc_step_record;
 
 /* constants ---------------------------------------------------------------- */
@@ -175,6 +176,23 @@ void orc_mc_steps(const orc_system *s, const orc_forcefield *ff, orc_replica *r,
                   const uint32_t key[2], int nsteps, int mode, orc_step_record *rec);
 /* Metropolis test shared by both modes (Replica.cpp:438-474, FP64). */
 int orc_metropolis(double delta, double temperature, double u, double *margin);
+
+/* Force-biased (smart) MC, Rossky, Doll & Friedman, J. Chem. Phys. 69, 4628
+ * (1978), for rigid bodies.  Each step picks a body and a move type from its
+ * Philox counter (tag 0x46420000) and draws three normals n (Box-Muller).
+ * A translation proposes d = beta A_t F + sqrt(2 A_t) n (F = force on the
+ * body), a rotation the rotation vector theta = beta A_r tau + sqrt(2 A_r) n
+ * (tau = torque about the centre).  The move is accepted with probability
+ * min(1, exp(-beta dE) q(new -> old) / q(old -> new)), the Gaussian proposal
+ * densities evaluated with the force (torque) at each end, so the chain keeps
+ * detailed balance with respect to exp(-beta E).  A_t, A_r in A^2 and rad^2;
+ * beta = 4184 / (R T).  With x the acceptance exponent (accept if x < 0 or
+ * x < -ln u), the record's margin is -x if x < 0, else min(x, |x + ln u|). */
+void orc_body_force(const orc_system *s, const orc_forcefield *ff, const float *x, const float *y, const float *z,
+                    int m, const float *bx, const float *by, const float *bz, const float *brx, const float *bry,
+                    const float *brz, double ft[6]);
+void orc_mc_fb_steps(const orc_system *s, const orc_forcefield *ff, orc_replica *r, const uint32_t key[2],
+                     int nsteps, double a_t, double a_r, int hastings, orc_step_record *rec);
 
 /* Exchange over temperature slots.  slot_replica[k] = replica holding the k-th
  * temperature of the sorted ladder ladder[k]; energies by replica index.

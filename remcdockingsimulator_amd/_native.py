@@ -64,6 +64,7 @@ SIGNATURES = {
     "remc_energy_noncrowder_all": (ctypes.c_int, [_vp, _dp]),
     "remc_mc_steps": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_mc_steps_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "remc_mc_steps_fb": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "remc_exchange": (ctypes.c_int, [_vp, _ip]),
     "remc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "remc_set_recording": (ctypes.c_int, [_vp, ctypes.c_int]),

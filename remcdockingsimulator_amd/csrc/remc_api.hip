@@ -293,6 +293,7 @@ int finalize(remc_ctx *c)
         d.tstep = h.tstep;
         d.rot_ch = std::cos(h.rstep / 2.0);
         d.rot_sh = std::sin(h.rstep / 2.0);
+        d.rstep = h.rstep;
         drep[r] = d;
         c->rep_bead_off[r] = rbo;
         c->rep_body_off[r] = rmo;
@@ -599,7 +600,7 @@ int autotune_mc(remc_ctx *c)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int trial_steps = 64;
+    const int trial_steps = 256;
     for (int bi = 0; bi < 3; ++bi) {
         const int blk = blocks[bi];
         if (c->mc_block && blk != c->mc_block) continue;
@@ -608,24 +609,30 @@ int autotune_mc(remc_ctx *c)
         if (rc != REMC_OK) return rc;
         for (int coop = 1; coop <= cmax; coop *= 2) {
             if (c->mc_coop && coop != c->mc_coop) continue;
-            c->synced_copies = 1;
-            rc = mc_launch(c, p, 4, blk, coop); // warm
-            if (rc != REMC_OK) return rc;
-            HIP_TRY(hipEventRecord(e0, c->stream));
-            rc = mc_launch(c, p, trial_steps, blk, coop);
-            if (rc != REMC_OK) return rc;
-            HIP_TRY(hipEventRecord(e1, c->stream));
-            HIP_TRY(hipEventSynchronize(e1));
-            rc = check_err(c);
-            if (rc != REMC_OK) return rc;
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best) {
-                best = ms;
+            // every trial starts from the same saved state, so each one makes the same moves;
+            // the best of two repeats discounts launch and clock noise
+            float ms_best = 1e30f;
+            for (int rep = 0; rep < 2; ++rep) {
+                c->synced_copies = 1;
+                rc = mc_launch(c, p, 4, blk, coop); // warm
+                if (rc != REMC_OK) return rc;
+                HIP_TRY(hipEventRecord(e0, c->stream));
+                rc = mc_launch(c, p, trial_steps, blk, coop);
+                if (rc != REMC_OK) return rc;
+                HIP_TRY(hipEventRecord(e1, c->stream));
+                HIP_TRY(hipEventSynchronize(e1));
+                rc = check_err(c);
+                if (rc != REMC_OK) return rc;
+                float ms = 0.f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                ms_best = std::min(ms_best, ms);
+                for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+            }
+            if (ms_best < best) {
+                best = ms_best;
                 best_block = blk;
                 best_coop = coop;
             }
-            for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
         }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -909,6 +916,32 @@ int remc_mc_steps(remc_ctx *c, int nsteps)
         rc = mc_launch(c, c->params(), nsteps, c->mc_block, c->mc_coop);
     }
     return rc;
+}
+
+int remc_mc_steps_fb(remc_ctx *c, int nsteps, int block)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
+    if (block == 0) block = 256;
+    if (block != 256 && block != 512 && block != 1024)
+        return fail(REMC_ERR_ARG, "workgroup size %d (256, 512 or 1024)", block);
+    if (nsteps == 0) return REMC_OK;
+    // copy 0 of every replica is current after any MC launch; the others go stale here
+    int rc = sync_copies(c, 1);
+    if (rc != REMC_OK) return rc;
+    KParams p = c->params();
+    p.lds_mb = std::max(1, c->max_body_beads);
+    const size_t lds = mc_fb_dynamic_lds(p);
+    if ((long long)lds + 8 * 1024 > c->lds_per_block)
+        return fail(REMC_ERR_LIMIT, "a body of %d beads does not fit the force-biased kernel's LDS staging",
+                    c->max_body_beads);
+    {
+        TimedLaunch t(c, 0);
+        HIP_TRY(launch_mc_fb(p, (int)c->reps.size(), nsteps, block, c->stream));
+    }
+    c->synced_copies = 1;
+    return REMC_OK;
 }
 
 int remc_mc_steps_plain(remc_ctx *c, int replica, int nsteps, int block)

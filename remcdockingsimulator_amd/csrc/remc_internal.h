@@ -35,6 +35,7 @@ struct DevRep {
     uint32_t rng_id;
     float tstep, pad;
     double rot_ch, rot_sh;
+    double rstep; // rotate step (rad)
 };
 
 // prescaled force field: etab[i*20+j] = lambda*(table-e0)*lj_cf*kbt_cf
@@ -99,6 +100,8 @@ struct KParams {
 // launchers (remc_kernels.hip)
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
 size_t mc_dynamic_lds(const KParams &p);
+hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s);
+size_t mc_fb_dynamic_lds(const KParams &p);
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
                               int noncrowder_only, int write_E, hipStream_t s);

@@ -889,6 +889,317 @@ static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, 
     return hipGetLastError();
 }
 
+// --------------------------------------------------------------------------
+// force-biased (smart) MC, Rossky, Doll & Friedman (1978), for rigid bodies
+//
+// One workgroup per replica.  Each step: draws (every thread, identical);
+// force and torque on the chosen body at its old position (pass A); the
+// proposal d = beta A F + sqrt(2A) n for a translation, the rotation vector
+// theta = beta A tau + sqrt(2A) n for a rotation (thread 0); staging of the
+// new position; dE and the force or torque at the new position (pass B); the
+// acceptance exponent x = beta dE + (|-d - beta A F'|^2 - |d - beta A F|^2)/(4A)
+// (accept if x < -ln u); commit.  A = t^2/6 for translations and r^2/6 for
+// rotations, from the replica's translate and rotate steps, so the random
+// part of a move has the plain move's RMS size.  Pair energies in FP32 with
+// Kahan sums of (new - old), forces and torques accumulated in FP64.
+// --------------------------------------------------------------------------
+
+// Energy of one pair (pair_lj_dh / pair_crowder, the same operations) and
+// g = (dE/dr)/r, so that the gradient of the pair energy with respect to the
+// moved bead is g (x_i - x_j).
+__device__ __forceinline__ void pair_eg(float r2, bool crow, float eij, float sij2, float qq, float inv_xi,
+                                        const DevFF &ff, float &e, float &g)
+{
+    if (crow) {
+        e = pair_crowder(r2, ff);
+        const float rc = __builtin_amdgcn_rcpf(fmaxf(r2, 1e-36f));
+        g = fmaxf(-6.0f * e * rc, -1e30f); // d(C r^-6)/dr / r = -6 E / r^2
+        return;
+    }
+    e = pair_lj_dh(r2, eij, sij2, qq, ff);
+    r2 = fmaxf(r2, 1e-36f);
+    const float rinv = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * rinv;
+    const float s2 = sij2 * (rinv * rinv);
+    const float x6 = s2 * s2 * s2;
+    const float dbase = 4.0f * (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv); // d[4 x6 (x6 - 1)]/dr
+    const bool flip = (eij > 0.0f) & (x6 > 0.5f);
+    const float dlj = flip ? eij * dbase : -eij * dbase;
+    const float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+    const float ddh = -dh * (inv_xi + rinv);
+    g = fmaxf(fminf((dlj + ddh) * rinv, 1e30f), -1e30f);
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
+{
+    constexpr int NW = BLOCK / 64;
+    __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
+    __shared__ double s_red[7][NW];
+    __shared__ double s_ft[7];
+    __shared__ double s_disp[3], s_q[4], s_coef[9];
+    __shared__ float s_cen[3];
+    __shared__ int s_dec;
+    __shared__ double s_dE;
+    extern __shared__ float4 smem[];
+    float4 *s_old = smem;                // old position, q_i c_dh
+    float4 *s_orel = s_old + p.lds_mb;   // old relative position, sigma
+    float4 *s_new = s_orel + p.lds_mb;   // new position
+    float4 *s_nrel = s_new + p.lds_mb;   // new relative position
+    int *s_tc = (int *)(s_nrel + p.lds_mb);
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = blockIdx.x;
+    const DevRep R = p.rep[r];
+    const DevSys S = p.sys[R.sys];
+    const DevFF ff = p.ff;
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
+    const double beta = remc_beta(p.T[r]);
+    const double a_t = (double)R.tstep * (double)R.tstep / 6.0, a_r = R.rstep * R.rstep / 6.0;
+    const float inv_xi = -ff.k_dh * 0.69314718f; // 1/xi from k_dh = -log2(e)/xi
+    unsigned long long step = p.step[r];
+    double E = p.E[r];
+    unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
+    float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
+    float *grx = p.rx + R.bead_off, *gry = p.ry + R.bead_off, *grz = p.rz + R.bead_off;
+    float *gcx = p.cx + R.body_off, *gcy = p.cy + R.body_off, *gcz = p.cz + R.body_off;
+    double *gq = p.quat + 4 * (size_t)R.body_off;
+    const int *tcs = p.tc + S.bead_off;
+    const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
+    const int N = S.n_beads;
+    __syncthreads();
+
+    // block sum of v[0..n) into s_ft (FP64, fixed order)
+    auto block_sum = [&](double *v, int n) {
+        for (int k = 0; k < n; ++k) {
+            const double t = wave_sum_dpp(v[k]);
+            if (lane == 0) s_red[k][wave] = t;
+        }
+        __syncthreads();
+        if (tid < n) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += s_red[tid][w];
+            s_ft[tid] = t;
+        }
+        __syncthreads();
+    };
+
+    for (int it = 0; it < nsteps; ++it, ++step) {
+        const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
+        const remc_fb_draws d = remc_fb_step_draws(step, R.rng_id, p.key0, p.key1, S.n_bodies);
+        const int body = d.body;
+        const int m0 = p.bstart[S.body_off + body], m1 = p.bend[S.body_off + body], nm = m1 - m0;
+        const bool crow_m = p.bcrow[S.body_off + body] != 0;
+        for (int k = tid; k < nm; k += BLOCK) {
+            const int b = m0 + k;
+            s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
+            s_orel[k] = make_float4(grx[b], gry[b], grz[b], sgs[b]);
+            s_tc[k] = tcs[b];
+        }
+        __syncthreads();
+
+        // pass A: force and torque at the old position
+        double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        for (int j = tid; j < N; j += BLOCK) {
+            if (j >= m0 && j < m1) continue;
+            const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
+            const int tcj = tcs[j];
+            const bool crow = crow_m || (tcj >> 8) != 0;
+            for (int k = 0; k < nm; ++k) {
+                const float4 o = s_old[k], orl = s_orel[k];
+                const float dx = min_image(o.x - xj, S.box, S.inv_box);
+                const float dy = min_image(o.y - yj, S.box, S.inv_box);
+                const float dz = min_image(o.z - zj, S.box, S.inv_box);
+                const float sij = 0.5f * (orl.w + sj);
+                float e, g;
+                pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow,
+                        s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, o.w * qj, inv_xi, ff, e, g);
+                const double fx = -(double)(g * dx), fy = -(double)(g * dy), fz = -(double)(g * dz);
+                v[0] += fx; v[1] += fy; v[2] += fz;
+                v[3] += (double)orl.y * fz - (double)orl.z * fy;
+                v[4] += (double)orl.z * fx - (double)orl.x * fz;
+                v[5] += (double)orl.x * fy - (double)orl.y * fx;
+            }
+        }
+        block_sum(v, 6);
+
+        // proposal (thread 0)
+        if (tid == 0) {
+            const double A = d.rotate ? a_r : a_t;
+            const double *fo = d.rotate ? s_ft + 3 : s_ft;
+            double disp[3];
+            for (int k = 0; k < 3; ++k) disp[k] = beta * A * fo[k] + sqrt(2.0 * A) * d.n[k];
+            const float cx = gcx[body], cy = gcy[body], cz = gcz[body];
+            if (d.rotate) {
+                const double ang = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+                double q[4] = {1.0, 0.0, 0.0, 0.0};
+                if (ang > 0.0) {
+                    double sh, ch;
+                    remc_sincos_turns(0.5 * ang / 6.283185307179586476925, &sh, &ch);
+                    q[0] = ch; q[1] = sh * disp[0] / ang; q[2] = sh * disp[1] / ang; q[3] = sh * disp[2] / ang;
+                    const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+                    for (int k = 0; k < 4; ++k) q[k] = q[k] / qm;
+                }
+                double cf[9];
+                remc_qcoeffs(q, cf);
+                for (int k = 0; k < 4; ++k) s_q[k] = q[k];
+                for (int k = 0; k < 9; ++k) s_coef[k] = cf[k];
+                s_cen[0] = cx; s_cen[1] = cy; s_cen[2] = cz;
+            } else {
+                const float t0 = (float)disp[0], t1 = (float)disp[1], t2 = (float)disp[2];
+                disp[0] = t0; disp[1] = t1; disp[2] = t2; // the displacement actually made
+                float px, py, pz;
+                remc_wrap_centre(cx, cy, cz, t0, t1, t2, S.box, &px, &py, &pz);
+                s_cen[0] = px; s_cen[1] = py; s_cen[2] = pz;
+            }
+            for (int k = 0; k < 3; ++k) s_disp[k] = disp[k];
+        }
+        __syncthreads();
+        for (int k = tid; k < nm; k += BLOCK) {
+            const float4 orl = s_orel[k];
+            float nrx = orl.x, nry = orl.y, nrz = orl.z;
+            if (d.rotate) remc_qrotate(s_coef, orl.x, orl.y, orl.z, &nrx, &nry, &nrz);
+            s_nrel[k] = make_float4(nrx, nry, nrz, 0.0f);
+            s_new[k] = make_float4(nrx + s_cen[0], nry + s_cen[1], nrz + s_cen[2], 0.0f);
+        }
+        __syncthreads();
+
+        // pass B: dE and the force and torque at the new position
+        float acc = 0.0f, comp = 0.0f;
+        for (int k = 0; k < 6; ++k) v[k] = 0.0;
+        for (int j = tid; j < N; j += BLOCK) {
+            if (j >= m0 && j < m1) continue;
+            const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
+            const int tcj = tcs[j];
+            const bool crow = crow_m || (tcj >> 8) != 0;
+            for (int k = 0; k < nm; ++k) {
+                const float4 o = s_old[k], orl = s_orel[k], n = s_new[k], nrl = s_nrel[k];
+                const float eij = s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)];
+                const float sij = 0.5f * (orl.w + sj);
+                const float qq = o.w * qj;
+                float eo, en, go, g;
+                {
+                    const float dx = min_image(o.x - xj, S.box, S.inv_box);
+                    const float dy = min_image(o.y - yj, S.box, S.inv_box);
+                    const float dz = min_image(o.z - zj, S.box, S.inv_box);
+                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi,
+                            ff, eo, go);
+                }
+                const float dx = min_image(n.x - xj, S.box, S.inv_box);
+                const float dy = min_image(n.y - yj, S.box, S.inv_box);
+                const float dz = min_image(n.z - zj, S.box, S.inv_box);
+                pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi, ff,
+                        en, g);
+                kahan_add(acc, comp, en - eo);
+                const double fx = -(double)(g * dx), fy = -(double)(g * dy), fz = -(double)(g * dz);
+                v[0] += fx; v[1] += fy; v[2] += fz;
+                v[3] += (double)nrl.y * fz - (double)nrl.z * fy;
+                v[4] += (double)nrl.z * fx - (double)nrl.x * fz;
+                v[5] += (double)nrl.x * fy - (double)nrl.y * fx;
+            }
+        }
+        v[6] = (double)acc - (double)comp;
+        // the old force and torque are needed again after the reduction reuses s_ft
+        double fo[3] = {0.0, 0.0, 0.0};
+        if (tid == 0) {
+            const int o3 = d.rotate ? 3 : 0;
+            fo[0] = s_ft[o3]; fo[1] = s_ft[o3 + 1]; fo[2] = s_ft[o3 + 2];
+        }
+        block_sum(v, 7);
+        if (tid == 0) {
+            const double A = d.rotate ? a_r : a_t;
+            const double *fn = d.rotate ? s_ft + 3 : s_ft;
+            double fwd = 0.0, rev = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const double a = s_disp[k] - beta * A * fo[k], b = -s_disp[k] - beta * A * fn[k];
+                fwd += a * a;
+                rev += b * b;
+            }
+            const double dE = s_ft[6];
+            const double x = beta * dE + (rev - fwd) / (4.0 * A);
+            s_dec = remc_metropolis_l(x, 1.0, d.nlu);
+            s_dE = dE;
+        }
+        __syncthreads();
+        const int dec = s_dec;
+        const double dE = s_dE;
+        if (dec) {
+            for (int k = tid; k < nm; k += BLOCK) {
+                const int b = m0 + k;
+                const float4 n = s_new[k];
+                gx[b] = n.x;
+                gy[b] = n.y;
+                gz[b] = n.z;
+                if (d.rotate) {
+                    const float4 nrl = s_nrel[k];
+                    grx[b] = nrl.x;
+                    gry[b] = nrl.y;
+                    grz[b] = nrl.z;
+                }
+            }
+            if (tid == 0) {
+                if (d.rotate) {
+                    double qo[4] = {gq[4 * body], gq[4 * body + 1], gq[4 * body + 2], gq[4 * body + 3]}, qn[4];
+                    remc_qmul(s_q, qo, qn);
+                    for (int k = 0; k < 4; ++k) gq[4 * body + k] = qn[k];
+                } else {
+                    gcx[body] = s_cen[0];
+                    gcy[body] = s_cen[1];
+                    gcz[body] = s_cen[2];
+                }
+            }
+            E += dE;
+            if (dec == 1) ++n_acc; else ++n_accb;
+        } else {
+            ++n_rej;
+        }
+        if (p.rec && tid == 0) {
+            const unsigned long long rec_start = p.rec_start[r];
+            const unsigned long long idx = step - rec_start;
+            if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
+                remc_step_record rr;
+                rr.body = body;
+                rr.move = d.rotate ? 1 : 0;
+                rr.accepted = dec;
+                rr.ticks = (int32_t)(wall_clock64() - t_step);
+                rr.delta = dE;
+                p.rec[(size_t)r * p.rec_max + idx] = rr;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        p.E[r] = E;
+        p.step[r] = step;
+        p.cnt[3 * r + 0] += n_acc;
+        p.cnt[3 * r + 1] += n_accb;
+        p.cnt[3 * r + 2] += n_rej;
+    }
+}
+
+size_t mc_fb_dynamic_lds(const KParams &p) { return (4 * sizeof(float4) + sizeof(int)) * (size_t)p.lds_mb; }
+
+template <int BLOCK>
+static hipError_t launch_mc_fb_t(const KParams &p, int n_rep, int nsteps, hipStream_t s)
+{
+    const size_t lds = mc_fb_dynamic_lds(p);
+    hipError_t e = hipFuncSetAttribute((const void *)k_mc_fb<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_mc_fb<BLOCK>), dim3(n_rep), dim3(BLOCK), lds, s, p, nsteps);
+    return hipGetLastError();
+}
+
+hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s)
+{
+    if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
+    switch (block) {
+    case 256: return launch_mc_fb_t<256>(p, n_rep, nsteps, s);
+    case 512: return launch_mc_fb_t<512>(p, n_rep, nsteps, s);
+    case 1024: return launch_mc_fb_t<1024>(p, n_rep, nsteps, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 // kernel variant: 0 = no partner cache, 1 = partner cache, 2 = speculative windows
 static int mc_variant(const KParams &p) { return p.lds_cl <= 0 ? 0 : p.spec ? 2 : 1; }
 

@@ -277,6 +277,71 @@ REMC_HD int remc_metropolis(double delta, double temperature, double u)
     return remc_metropolis_l(delta, remc_beta(temperature), remc_neglog_u(u));
 }
 
+// sin and cos of 2 pi t in FP64 for finite t: t reduced to f = t - rint(t),
+// then to r = f - k/4 with |r| <= 1/8 (both exact), x = 2 pi r, |x| <= pi/4,
+// and the Taylor series of sin to x^15 and of cos to x^16 (truncation below
+// 1e-16), rotated by the quadrant k.
+REMC_HD void remc_sincos_turns(double t, double *sn, double *cs)
+{
+    const double f = t - rint(t);
+    const double k = rint(4.0 * f);
+    const double x = (f - 0.25 * k) * 6.283185307179586476925;
+    const double x2 = x * x;
+    double ps = -1.0 / 1307674368000.0;
+    ps = fma(ps, x2, 1.0 / 6227020800.0);
+    ps = fma(ps, x2, -1.0 / 39916800.0);
+    ps = fma(ps, x2, 1.0 / 362880.0);
+    ps = fma(ps, x2, -1.0 / 5040.0);
+    ps = fma(ps, x2, 1.0 / 120.0);
+    ps = fma(ps, x2, -1.0 / 6.0);
+    const double sx = fma(x * x2, ps, x);
+    double pc = 1.0 / 20922789888000.0;
+    pc = fma(pc, x2, -1.0 / 87178291200.0);
+    pc = fma(pc, x2, 1.0 / 479001600.0);
+    pc = fma(pc, x2, -1.0 / 3628800.0);
+    pc = fma(pc, x2, 1.0 / 40320.0);
+    pc = fma(pc, x2, -1.0 / 720.0);
+    pc = fma(pc, x2, 1.0 / 24.0);
+    pc = fma(pc, x2, -0.5);
+    const double cx = fma(x2, pc, 1.0);
+    switch (((int)k) & 3) {
+    case 0: *sn = sx; *cs = cx; break;
+    case 1: *sn = cx; *cs = -sx; break;
+    case 2: *sn = -sx; *cs = -cx; break;
+    default: *sn = -cx; *cs = sx; break;
+    }
+}
+
+// Draws of a force-biased MC step: body, move type, three standard normals
+// (Box-Muller on (1 - u1, u2) and (1 - u3, u4)) and -ln u of the acceptance
+// uniform, from the step's two Philox blocks under the tag 0x46420000.
+#define REMC_TAG_FB 0x46420000u
+struct remc_fb_draws {
+    int body;
+    int rotate;
+    double n[3];
+    double nlu;
+};
+
+REMC_HD remc_fb_draws remc_fb_step_draws(uint64_t step, uint32_t rng_id, uint32_t k0, uint32_t k1, int n_bodies)
+{
+    remc_u32x4 w0 = remc_philox((uint32_t)step, (uint32_t)(step >> 32), rng_id, REMC_TAG_FB, k0, k1);
+    remc_u32x4 w1 = remc_philox((uint32_t)step, (uint32_t)(step >> 32), rng_id, REMC_TAG_FB + 1u, k0, k1);
+    remc_fb_draws d;
+    d.body = (int)(((uint64_t)w0.v[0] * (uint64_t)n_bodies) >> 32);
+    d.rotate = remc_u01(w0.v[1]) < 0.5;
+    const double rho1 = sqrt(2.0 * remc_neglog_u(1.0 - remc_u01(w0.v[2])));
+    const double rho2 = sqrt(2.0 * remc_neglog_u(1.0 - remc_u01(w1.v[0])));
+    double s2, c2, s4, c4;
+    remc_sincos_turns(remc_u01(w0.v[3]), &s2, &c2);
+    remc_sincos_turns(remc_u01(w1.v[1]), &s4, &c4);
+    d.n[0] = rho1 * c2;
+    d.n[1] = rho1 * s2;
+    d.n[2] = rho2 * c4;
+    d.nlu = remc_neglog_u(remc_u01(w1.v[2]));
+    return d;
+}
+
 // Exchange test of ladder rungs (Ti, Tj) holding energies (Ei, Ej)
 // (main.cpp:1440-1441).
 REMC_HD int remc_exchange_accept(double Ti, double Tj, double Ei, double Ej, double u)

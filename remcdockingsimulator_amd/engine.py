@@ -120,6 +120,10 @@ class Engine:
     def mc_steps(self, nsteps):
         self._chk(self.lib.remc_mc_steps(self.ctx, int(nsteps)), "remc_mc_steps")
 
+    def mc_steps_fb(self, nsteps, block=0):
+        """Force-biased (smart) MC steps on every replica (remc_mc_steps_fb)."""
+        self._chk(self.lib.remc_mc_steps_fb(self.ctx, int(nsteps), int(block)), "remc_mc_steps_fb")
+
     def mc_steps_plain(self, replica, nsteps, block=0):
         self._chk(self.lib.remc_mc_steps_plain(self.ctx, replica, int(nsteps), block), "remc_mc_steps_plain")
 
