@@ -103,28 +103,44 @@ __device__ __forceinline__ double slot_value(unsigned long long w)
     return __longlong_as_double((long long)(w & ~SLOT_TAG_MASK));
 }
 
-// log2 of the number of k-groups (power of two <= waves) that balances the
-// (moved bead x partner bead) work of one step across the workgroup.  All
-// divisions are by powers of two (shifts).
-template <int BLOCK>
-__device__ __forceinline__ int pick_kgroups_log2(int nm, int nj)
+// floor(a / b) and ceil(a / b) for 0 <= a < 2^22, 0 < b < 2^22: the float
+// quotient is within one of the true one, and one correction step each way
+// makes it exact.
+__device__ __forceinline__ int div_floor_small(int a, int b)
 {
-    constexpr int NW = BLOCK / 64;
-    constexpr int LB = BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10;
-    // the critical path of a step is the busiest thread: ceil(nj/G) partner
-    // beads times ceil(nm/tk) moved beads; take the smallest product
-    int best = 0, best_work = 0x7fffffff;
-    for (int lt = 0; (1 << lt) <= NW && (1 << lt) <= nm; ++lt) {
-        const int lg = LB - lt; // log2 of the group width
-        const int jper = (nj + (1 << lg) - 1) >> lg;
-        const int kper = (nm + (1 << lt) - 1) >> lt;
-        const int work = jper * kper;
-        if (work < best_work) {
-            best_work = work;
-            best = lt;
+    int q = (int)((float)a * __builtin_amdgcn_rcpf((float)b));
+    if (q * b > a) --q;
+    if ((q + 1) * b <= a) ++q;
+    return q;
+}
+
+__device__ __forceinline__ int div_ceil_small(int a, int b) { return div_floor_small(a + b - 1, b); }
+
+// Work split of one step's (moved bead x partner bead) pairs over a
+// workgroup: groups of G lanes, lane l of group g takes partners l, l+G, ...
+// (t = ceil(nj/G) of them) against moved beads [g*kper, (g+1)*kper).  Of the
+// splits with t = ceil(nj/BLOCK) .. +3 j-iterations per lane, the one with the
+// fewest pairs per thread (t * kper) is taken.
+template <int BLOCK>
+__device__ __forceinline__ void pick_split(int nm, int nj, int &G, int &kper)
+{
+    G = 1;
+    kper = nm;
+    if (nj <= 0) return;
+    const int t1 = div_ceil_small(nj, BLOCK);
+    int best = 0x7fffffff;
+#pragma unroll
+    for (int dt = 0; dt < 4; ++dt) {
+        const int t = t1 + dt;
+        const int Gt = div_ceil_small(nj, t);
+        const int groups = div_floor_small(BLOCK, Gt);
+        const int kp = div_ceil_small(nm, groups);
+        if (t * kp < best) {
+            best = t * kp;
+            G = Gt;
+            kper = kp;
         }
     }
-    return best;
 }
 
 // Draws of one MC step and the move parameters that depend on them only
@@ -209,7 +225,6 @@ template <int BLOCK, bool CACHE, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
-    constexpr int LB = BLOCK == 256 ? 8 : BLOCK == 512 ? 9 : 10;
     constexpr int WS = SPEC ? NW * SPEC_MAX : 1;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[NW];
@@ -687,16 +702,18 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
         const int js = split ? cjs : 0, je = split ? cje : N;
         // this member's share of dE = sum_{i in body} sum_{j in slice, not in body} e(new_i,j) - e(old_i,j),
-        // split into tk groups of moved beads x G lanes over partner beads
-        const int ov = max(0, min(je, m1) - max(js, m0));
-        const int lt = pick_kgroups_log2<BLOCK>(nm, (je - js) - ov);
-        const int lgw = LB - lt;
-        const int G = 1 << lgw;
-        const int g = tid >> lgw, l = tid & (G - 1);
-        const int k0 = (g * nm) >> lt, k1 = ((g + 1) * nm) >> lt;
+        // split into groups of moved beads x G lanes over the slice's partner
+        // beads (the slice without the moved body's overlap [ovs, ovs + ov))
+        const int ovs = max(js, m0);
+        const int ov = max(0, min(je, m1) - ovs);
+        const int nj = (je - js) - ov;
+        int G, kper;
+        pick_split<BLOCK>(nm, nj, G, kper);
+        const int g = div_floor_small(tid, G), l = tid - g * G;
+        const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         float acc = 0.0f, comp = 0.0f;
-        for (int j = js + l; j < je; j += G) {
-            if (j >= m0 && j < m1) continue;
+        for (int jj = l; jj < nj; jj += G) {
+            const int j = js + jj + (js + jj >= ovs ? ov : 0);
             float xj, yj, zj, qj, sj;
             int tcj;
             if (CACHE) {
