@@ -122,6 +122,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--block", type=int, default=0, help="MC workgroup size (0 = autotune)")
     ap.add_argument("--coop", type=int, default=0, help="MC workgroups per replica (0 = autotune)")
+    ap.add_argument("--fb-steps", type=int, default=2000,
+                    help="force-biased MC steps per replica timed after the main run (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -230,6 +232,27 @@ def main():
                  "steps": args.plain_steps}
         log("plain: %.3f s for %d steps" % (pw, args.plain_steps))
 
+    # ---------------------------------------------------- force-biased ----
+    fb = None
+    if args.fb_steps > 0 and len(wl.systems) == 1:
+        fb_first = int(e.counters()[0, 3])
+        e.timing_enable(True)
+        e.synchronize()
+        t4 = time.perf_counter()
+        e.mc_steps_fb(args.fb_steps)
+        e.synchronize()
+        t5 = time.perf_counter()
+        fms, fnl = e.timing(Engine.MC)
+        fw = t5 - t4
+        fpairs = rngm.delta_pairs(SEED, range(R), fb_first, args.fb_steps, sizes[0], tag=rngm.TAG_FB)
+        fb = {"mc_steps_per_s": R * args.fb_steps / fw,
+              "dE_pair_interactions_per_s": fpairs / fw,
+              "kernel": "k_mc_fb (force-biased MC, one 512-thread workgroup per replica)",
+              "kernel_avg_ms": fms / max(fnl, 1), "launches": fnl, "steps": args.fb_steps,
+              "work_per_step": "force at the old position, energy and force at the new, energy at the old: "
+                               "2 n_m (N - n_m) pair energies and 2 n_m (N - n_m) pair derivatives"}
+        log("fb: %.3f s for %d steps" % (fw, args.fb_steps))
+
     # ------------------------------------------------------------ cpu ----
     cpu = None
     if not args.no_cpu and len(wl.systems) == 1 and gpu_records is not None:
@@ -261,6 +284,7 @@ def main():
         "speedup_vs_plain": {"pair_interactions": value / plain["pair_interactions_per_s"],
                              "mc_steps": (steps_total / wall) / plain["mc_steps_per_s"]} if plain else None,
         "cpu_baseline": cpu,
+        "force_biased": fb,
         "energy_mean": float(np.mean(energies)),
     }
     e.close()

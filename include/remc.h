@@ -157,7 +157,7 @@ REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int blo
  * uniform proposal of Replica::MCSearch (Replica.cpp:386-420) with one driven
  * by the gradient that remc_adjoint returns.  Shares the replicas, energies,
  * step and decision counters with remc_mc_steps.  block: workgroup size (256,
- * 512 or 1024; 0 = 256). */
+ * 512 or 1024; 0 = 512). */
 REMC_API int remc_mc_steps_fb(remc_ctx *ctx, int nsteps, int block);
 /* One replica-exchange round over every system's temperature ladder, pairs
  * (k,k+1) for k = offset, offset+2, ...; the offset alternates per call

@@ -923,7 +923,7 @@ int remc_mc_steps_fb(remc_ctx *c, int nsteps, int block)
     CHECK_CTX(c);
     FINALIZE(c);
     if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
-    if (block == 0) block = 256;
+    if (block == 0) block = 512;
     if (block != 256 && block != 512 && block != 1024)
         return fail(REMC_ERR_ARG, "workgroup size %d (256, 512 or 1024)", block);
     if (nsteps == 0) return REMC_OK;

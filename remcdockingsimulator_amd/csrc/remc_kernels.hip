@@ -1015,23 +1015,29 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         }
         __syncthreads();
 
-        // pass A: force and torque at the old position
+        // pass A: force and torque at the old position; the (moved bead x
+        // partner bead) pairs split over the workgroup as in the MC kernel
+        const int nj = N - nm;
+        int G, kper;
+        pick_split<BLOCK>(nm, nj, G, kper);
+        const int g = div_floor_small(tid, G), l = tid - g * G;
+        const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
-        for (int j = tid; j < N; j += BLOCK) {
-            if (j >= m0 && j < m1) continue;
+        for (int jj = l; jj < nj; jj += G) {
+            const int j = jj + (jj >= m0 ? nm : 0);
             const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
             const int tcj = tcs[j];
             const bool crow = crow_m || (tcj >> 8) != 0;
-            for (int k = 0; k < nm; ++k) {
+            for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k];
                 const float dx = min_image(o.x - xj, S.box, S.inv_box);
                 const float dy = min_image(o.y - yj, S.box, S.inv_box);
                 const float dz = min_image(o.z - zj, S.box, S.inv_box);
                 const float sij = 0.5f * (orl.w + sj);
-                float e, g;
+                float e, g2;
                 pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow,
-                        s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, o.w * qj, inv_xi, ff, e, g);
-                const double fx = -(double)(g * dx), fy = -(double)(g * dy), fz = -(double)(g * dz);
+                        s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, o.w * qj, inv_xi, ff, e, g2);
+                const double fx = -(double)(g2 * dx), fy = -(double)(g2 * dy), fz = -(double)(g2 * dz);
                 v[0] += fx; v[1] += fy; v[2] += fz;
                 v[3] += (double)orl.y * fz - (double)orl.z * fy;
                 v[4] += (double)orl.z * fx - (double)orl.x * fz;
@@ -1081,20 +1087,20 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         }
         __syncthreads();
 
-        // pass B: dE and the force and torque at the new position
+        // pass B: dE and the force and torque at the new position (same split)
         float acc = 0.0f, comp = 0.0f;
         for (int k = 0; k < 6; ++k) v[k] = 0.0;
-        for (int j = tid; j < N; j += BLOCK) {
-            if (j >= m0 && j < m1) continue;
+        for (int jj = l; jj < nj; jj += G) {
+            const int j = jj + (jj >= m0 ? nm : 0);
             const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
             const int tcj = tcs[j];
             const bool crow = crow_m || (tcj >> 8) != 0;
-            for (int k = 0; k < nm; ++k) {
+            for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k], n = s_new[k], nrl = s_nrel[k];
                 const float eij = s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)];
                 const float sij = 0.5f * (orl.w + sj);
                 const float qq = o.w * qj;
-                float eo, en, go, g;
+                float eo, en, go, g2;
                 {
                     const float dx = min_image(o.x - xj, S.box, S.inv_box);
                     const float dy = min_image(o.y - yj, S.box, S.inv_box);
@@ -1106,9 +1112,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
                 const float dy = min_image(n.y - yj, S.box, S.inv_box);
                 const float dz = min_image(n.z - zj, S.box, S.inv_box);
                 pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi, ff,
-                        en, g);
+                        en, g2);
                 kahan_add(acc, comp, en - eo);
-                const double fx = -(double)(g * dx), fy = -(double)(g * dy), fz = -(double)(g * dz);
+                const double fx = -(double)(g2 * dx), fy = -(double)(g2 * dy), fz = -(double)(g2 * dz);
                 v[0] += fx; v[1] += fy; v[2] += fz;
                 v[3] += (double)nrl.y * fz - (double)nrl.z * fy;
                 v[4] += (double)nrl.z * fx - (double)nrl.x * fz;

@@ -10,6 +10,7 @@ import numpy as np
 M0, M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
 W0, W1 = np.uint32(0x9E3779B9), np.uint32(0xBB67AE85)
 TAG_MC = 0x4D430000
+TAG_FB = 0x46420000
 MASK = np.uint64(0xFFFFFFFF)
 
 
@@ -33,26 +34,28 @@ def philox4x32_10(c0, c1, c2, c3, k0, k1):
     return c0, c1, c2, c3
 
 
-def step_bodies(seed, rng_id, first_step, nsteps, n_bodies):
-    """Body index and move type (1 = rotate) of steps [first_step, first_step+nsteps)."""
+def step_bodies(seed, rng_id, first_step, nsteps, n_bodies, tag=TAG_MC):
+    """Body index and move type (1 = rotate) of steps [first_step, first_step+nsteps).
+    tag=TAG_MC: the uniform-proposal MC (body from word 0, move from word 1 of
+    the first block); tag=TAG_FB: the force-biased MC (the same two words)."""
     s = np.arange(first_step, first_step + nsteps, dtype=np.uint64)
     lo = (s & MASK).astype(np.uint32)
     hi = (s >> np.uint64(32)).astype(np.uint32)
-    w0, w1, _, _ = philox4x32_10(lo, hi, np.uint32(rng_id), np.uint32(TAG_MC), seed & 0xFFFFFFFF,
+    w0, w1, _, _ = philox4x32_10(lo, hi, np.uint32(rng_id), np.uint32(tag), seed & 0xFFFFFFFF,
                                  (seed >> 32) & 0xFFFFFFFF)
     body = ((w0.astype(np.uint64) * np.uint64(n_bodies)) >> np.uint64(32)).astype(np.int64)
     rotate = (w1.astype(np.float64) * (1.0 / 4294967296.0)) < 0.5
     return body, rotate
 
 
-def delta_pairs(seed, rng_ids, first_step, nsteps, body_sizes):
+def delta_pairs(seed, rng_ids, first_step, nsteps, body_sizes, tag=TAG_MC):
     """Pair terms evaluated by dE kernels (old and new positions) over a run:
     sum over replicas and steps of 2*n_m*(N - n_m)."""
     sizes = np.asarray(body_sizes, dtype=np.int64)
     n = int(sizes.sum())
     total = 0
     for r in rng_ids:
-        body, _ = step_bodies(seed, r, first_step, nsteps, len(sizes))
+        body, _ = step_bodies(seed, r, first_step, nsteps, len(sizes), tag)
         nm = sizes[body]
         total += int(np.sum(2 * nm * (n - nm)))
     return total
