@@ -523,6 +523,8 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
     } else {
         return;
     }
+    // speculative windows read the partner range (the replica, or the member's slice) from LDS
+    p.spec = c->mc_window ? 1 : 0;
     if (20LL * c->max_bodies <= budget) {
         p.lds_nbody = c->max_bodies;
         p.cache_bodies = 1;
@@ -538,8 +540,6 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
         p.cache_quat = 1;
         budget -= 32LL * c->max_bodies;
     }
-    // speculative windows need every bead and body of the replica in LDS
-    p.spec = c->mc_window && !p.cache_slice ? 1 : 0;
 }
 
 // Largest workgroups-per-replica count whose grid is co-resident.

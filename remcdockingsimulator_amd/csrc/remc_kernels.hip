@@ -152,7 +152,7 @@ struct StepDraw {
     float t[3];      // translation vector of a translate move
     int body;
     int rotate;
-    int pad;
+    int m0, nm; // the moved body's first bead and bead count
 };
 constexpr int DRAW_BATCH = 64;
 
@@ -232,7 +232,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ int s_abort;
     __shared__ StepDraw s_draw[DRAW_BATCH];
     __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
-    __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old/new position, new relative position
+    __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old (w: q c_dh) / new (w: sigma) position, new rel (w: q)
+    __shared__ int s_wtc[WS];                             // type | crowder << 8 of the staged beads
     __shared__ float4 s_wcen[SPEC ? NW : 1];                // new centre of a translated slot
     __shared__ double s_wq[SPEC ? 4 * NW : 1];              // new rotation of a rotated slot
     __shared__ double s_wbase[SPEC ? NW : 1], s_wdE[SPEC ? NW : 1];
@@ -350,12 +351,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     } else {
                         remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
                     }
-                    if (SPEC) {
-                        const int gb = S.body_off + dd.body;
-                        const int nmb = p.bend[gb] - p.bstart[gb];
-                        elig = nmb <= SPEC_MAX &&
-                               !(coop > 1 && (long long)nmb * (long long)(N - nmb) >= (long long)p.coop_min_pairs);
-                    }
+                    const int gb = S.body_off + dd.body;
+                    sd.m0 = p.bstart[gb];
+                    sd.nm = p.bend[gb] - sd.m0;
+                    if (SPEC) elig = sd.nm <= SPEC_MAX;
                 }
                 if (SPEC) {
                     const unsigned long long m = __ballot(elig);
@@ -383,19 +382,26 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (k < w) {
                     const StepDraw &Dk = s_draw[off + k];
                     const int bk = Dk.body;
-                    const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                    const int m0k = Dk.m0, nmk = Dk.nm;
                     if (i < nmk) {
+                        // the moved beads come from the partner cache when it holds
+                        // them (a member's slice may not), else from HBM
                         const int b = m0k + i;
+                        const bool inc = b >= clo && b < chi;
                         const float rx = cache_rel ? s_rx[b] : grx[b];
                         const float ry = cache_rel ? s_ry[b] : gry[b];
                         const float rz = cache_rel ? s_rz[b] : grz[b];
                         float nrx = rx, nry = ry, nrz = rz;
-                        float ncx = s_cx[bk], ncy = s_cy[bk], ncz = s_cz[bk];
+                        float ncx = cache_bodies ? s_cx[bk] : gcx[bk], ncy = cache_bodies ? s_cy[bk] : gcy[bk],
+                              ncz = cache_bodies ? s_cz[bk] : gcz[bk];
                         if (Dk.rotate) remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
                         else remc_wrap_centre(ncx, ncy, ncz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
-                        s_wold[tid] = make_float4(s_jx[b], s_jy[b], s_jz[b], s_jq[b] * ff.c_dh);
-                        s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, s_js[b]);
-                        s_wrel[tid] = make_float4(nrx, nry, nrz, 0.0f);
+                        const float qb = inc ? s_jq[b - clo] : qs[b];
+                        s_wold[tid] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
+                                          : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
+                        s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+                        s_wrel[tid] = make_float4(nrx, nry, nrz, qb);
+                        s_wtc[tid] = inc ? s_jtc[b - clo] : tcs[b];
                         if (i == 0) {
                             if (Dk.rotate) {
                                 double qo[4], qn[4];
@@ -419,26 +425,26 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             const int wjs = coop > 1 ? cjs : 0, wje = coop > 1 ? cje : N;
             if (wave < w) {
                 const int k = wave;
-                const int bk = s_draw[off + k].body;
-                const int m0k = s_bs[bk], m1k = s_be[bk];
-                const bool crow_m = (s_jtc[m0k] >> 8) != 0;
+                const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
+                const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
                 float acc = 0.0f, comp = 0.0f;
-                for (int i = 0; i < m1k - m0k; ++i) {
+                for (int i = 0; i < nmk; ++i) {
                     const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
-                    const float *erow = s_etab + (s_jtc[m0k + i] & 0xff) * REMC_NTYPES;
+                    const float *erow = s_etab + (s_wtc[k * SPEC_MAX + i] & 0xff) * REMC_NTYPES;
                     for (int j = wjs + lane; j < wje; j += 64) {
                         if (j >= m0k && j < m1k) continue;
-                        const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
-                        const int tcj = s_jtc[j];
+                        const int jc = j - clo; // the partner range lies in the cache
+                        const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                        const int tcj = s_jtc[jc];
                         float eo, en;
                         if (crow_m || (tcj >> 8) != 0) {
                             eo = pair_crowder(dist2(xj, yj, zj, o, S), ff);
                             en = pair_crowder(dist2(xj, yj, zj, n, S), ff);
                         } else {
                             const float eij = erow[tcj & 0xff];
-                            const float sij = 0.5f * (n.w + s_js[j]);
+                            const float sij = 0.5f * (n.w + s_js[jc]);
                             const float sij2 = sij * sij;
-                            const float qq = o.w * s_jq[j];
+                            const float qq = o.w * s_jq[jc];
                             eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
                             en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
                         }
@@ -462,13 +468,12 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     float cv = 0.0f;
                     bool touches = bj == bk; // the same body: slot kk's proposal depends on slot jj's move
                     if (bj != bk) {
-                        const int m0j = s_bs[bj], nmj = s_be[bj] - m0j;
-                        const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
-                        const bool crow = ((s_jtc[m0j] | s_jtc[m0k]) >> 8) != 0;
+                        const int nmj = s_draw[off + jj].nm, nmk = s_draw[off + kk].nm;
+                        const bool crow = ((s_wtc[jj * SPEC_MAX] | s_wtc[kk * SPEC_MAX]) >> 8) != 0;
                         float acc = 0.0f, comp = 0.0f;
                         for (int i = 0; i < nmk; ++i) {
                             const float4 o = s_wold[kk * SPEC_MAX + i], n = s_wnew[kk * SPEC_MAX + i];
-                            const int ti = s_jtc[m0k + i] & 0xff;
+                            const int ti = s_wtc[kk * SPEC_MAX + i] & 0xff;
                             for (int l = 0; l < nmj; ++l) {
                                 const float4 O = s_wold[jj * SPEC_MAX + l], Nn = s_wnew[jj * SPEC_MAX + l];
                                 float e_nn, e_on, e_no, e_oo;
@@ -478,11 +483,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                                     e_no = pair_crowder(dist2(O.x, O.y, O.z, n, S), ff);
                                     e_oo = pair_crowder(dist2(O.x, O.y, O.z, o, S), ff);
                                 } else {
-                                    const int bl = m0j + l;
-                                    const float eij = s_etab[ti * REMC_NTYPES + (s_jtc[bl] & 0xff)];
-                                    const float sij = 0.5f * (n.w + s_js[bl]);
+                                    const int sl = jj * SPEC_MAX + l;
+                                    const float eij = s_etab[ti * REMC_NTYPES + (s_wtc[sl] & 0xff)];
+                                    const float sij = 0.5f * (n.w + Nn.w);
                                     const float sij2 = sij * sij;
-                                    const float qq = o.w * s_jq[bl];
+                                    const float qq = o.w * s_wrel[sl].w;
                                     e_nn = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, n, S), eij, sij2, qq, ff);
                                     e_on = pair_lj_dh(dist2(Nn.x, Nn.y, Nn.z, o, S), eij, sij2, qq, ff);
                                     e_no = pair_lj_dh(dist2(O.x, O.y, O.z, n, S), eij, sij2, qq, ff);
@@ -594,16 +599,18 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (k < weff && s_wdec[k]) {
                     const StepDraw &Dk = s_draw[off + k];
                     const int bk = Dk.body;
-                    const int m0k = s_bs[bk], nmk = s_be[bk] - m0k;
+                    const int m0k = Dk.m0, nmk = Dk.nm;
                     if (i < nmk) {
                         const int b = m0k + i;
                         const float4 n = s_wnew[tid];
                         gx[b] = n.x;
                         gy[b] = n.y;
                         gz[b] = n.z;
-                        s_jx[b] = n.x;
-                        s_jy[b] = n.y;
-                        s_jz[b] = n.z;
+                        if (b >= clo && b < chi) {
+                            s_jx[b - clo] = n.x;
+                            s_jy[b - clo] = n.y;
+                            s_jz[b - clo] = n.z;
+                        }
                         if (Dk.rotate) {
                             const float4 nr = s_wrel[tid];
                             grx[b] = nr.x;
@@ -627,9 +634,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                                 gcx[bk] = nc.x;
                                 gcy[bk] = nc.y;
                                 gcz[bk] = nc.z;
-                                s_cx[bk] = nc.x;
-                                s_cy[bk] = nc.y;
-                                s_cz[bk] = nc.z;
+                                if (cache_bodies) {
+                                    s_cx[bk] = nc.x;
+                                    s_cy[bk] = nc.y;
+                                    s_cz[bk] = nc.z;
+                                }
                             }
                         }
                     }
@@ -1240,8 +1249,9 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
 {
     if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
     if (coop < 1 || coop > REMC_MAX_COOP) return hipErrorInvalidValue;
-    // the speculative kernel indexes its caches by bead and body: they must hold the whole replica
-    if (p.spec && (p.lds_cl <= 0 || p.cache_slice || !p.cache_bodies)) return hipErrorInvalidValue;
+    // the speculative kernel reads its partner range from the cache (the whole replica, or with
+    // every member on its own slice the member's slice)
+    if (p.spec && p.lds_cl <= 0) return hipErrorInvalidValue;
     if (coop > 1) {
         const int n = n_rep * REMC_SLOTS_PER_REP;
         hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
