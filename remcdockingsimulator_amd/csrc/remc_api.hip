@@ -325,13 +325,17 @@ int finalize(remc_ctx *c)
     const remc_forcefield &f = c->ff;
     const double lj_scale = f.lj_cf * f.kbt_cf;
     for (int i = 0; i < REMC_NTYPES * REMC_NTYPES; ++i)
-        etab[i] = (float)(f.lambda * ((double)f.table[i] - f.e0) * lj_scale);
+        etab[i] = 4.0f * (float)(f.lambda * ((double)f.table[i] - f.e0) * lj_scale); // 4 E_ij (exact scaling)
     c->devff.c_crow = (float)lj_scale;
     c->devff.crow_s2 = (float)(f.crowder_sigma * f.crowder_sigma);
     c->devff.crow_cut2 = (float)(f.crowder_cutoff * f.crowder_cutoff);
     c->devff.c_dh = (float)(f.dh_cf * f.kbt_cf);
     c->devff.k_dh = (float)(-1.4426950408889634 / f.xi);
     c->devff.eps_r = 1e-38f;
+    {
+        const double s2 = f.crowder_sigma * f.crowder_sigma;
+        c->devff.crow_k = (float)(lj_scale * s2 * s2 * s2);
+    }
 
     c->max_tiles = (c->max_beads + FULL_TILE - 1) / FULL_TILE;
 

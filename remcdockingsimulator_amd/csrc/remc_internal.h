@@ -46,7 +46,8 @@ struct DevFF {
     float c_dh;      // dh_cf*kbt_cf
     float k_dh;      // -log2(e)/xi, exp(-r/xi) = exp2(r*k_dh)
     float eps_r;     // 1e-38 added to r
-    float pad0, pad1;
+    float crow_k;    // c_crow * crow_s2^3: crowder term = crow_k / r^6
+    float pad1;
 };
 
 struct KParams {

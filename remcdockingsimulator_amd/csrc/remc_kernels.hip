@@ -34,29 +34,32 @@ __device__ __forceinline__ float min_image(float d, float box, float inv_box)
     return __builtin_fmaf(-box, __builtin_rintf(d * inv_box), d);
 }
 
-// Energy of one non-crowder pair from squared distance.
-__device__ __forceinline__ float pair_lj_dh(float r2, float eij, float sij2, float qq, const DevFF &ff)
+// Energy of one non-crowder pair from squared distance.  eij4 = 4 E_ij (the
+// table holds the factor 4, so LJ = E (4 x6 (x6 - 1)) costs one FMA: scaling by
+// 4 is exact, the result is the same bits as forming 4 x6 (x6 - 1) and E).
+__device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, float qq, const DevFF &ff)
 {
     r2 = fmaxf(r2, 1e-36f);
     float rinv = __builtin_amdgcn_rsqf(r2);
     float r = r2 * rinv;
     float s2 = sij2 * (rinv * rinv);
     float x6 = s2 * s2 * s2;
-    float x4 = 4.0f * x6;
-    float base = __builtin_fmaf(x4, x6, -x4);           // 4 x6 (x6 - 1)
-    bool flip = (eij > 0.0f) & (x6 > 0.5f);             // r < 2^(1/6) sigma
-    float lj = flip ? eij * (base + 2.0f) : -eij * base;
+    float base = __builtin_fmaf(x6, x6, -x6);           // x6 (x6 - 1)
+    bool flip = (eij4 > 0.0f) & (x6 > 0.5f);            // r < 2^(1/6) sigma
+    float lj = eij4 * (flip ? base + 0.5f : -base);     // E (LJ + 2) when flipped, else -E LJ
     float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
     // coincident beads overflow to inf (or 0*inf); cap so compensated sums stay finite
     return fminf(lj + dh, 1e30f);
 }
 
+// Crowder term C (sigma_c / r)^6 inside the cutoff as crow_k (1/r^2)^3.  r2 = 0
+// (coincident beads) gives inf, capped like the pair term; no clamp is needed
+// since no zero multiplies the infinity.
 __device__ __forceinline__ float pair_crowder(float r2, const DevFF &ff)
 {
-    r2 = fmaxf(r2, 1e-36f);
-    float t = ff.crow_s2 * __builtin_amdgcn_rcpf(r2);
-    float t3 = t * t * t;
-    return r2 < ff.crow_cut2 ? fminf(ff.c_crow * t3, 1e30f) : 0.0f;
+    const float rc = __builtin_amdgcn_rcpf(r2);
+    const float rc3 = rc * rc * rc;
+    return r2 < ff.crow_cut2 ? fminf(ff.crow_k * rc3, 1e30f) : 0.0f;
 }
 
 // Kahan step, kept as written (no reassociation is enabled for this file).
@@ -948,7 +951,7 @@ __device__ __forceinline__ void pair_eg(float r2, bool crow, float eij, float si
     const float r = r2 * rinv;
     const float s2 = sij2 * (rinv * rinv);
     const float x6 = s2 * s2 * s2;
-    const float dbase = 4.0f * (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv); // d[4 x6 (x6 - 1)]/dr
+    const float dbase = (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv); // d[x6 (x6 - 1)]/dr (eij holds 4 E)
     const bool flip = (eij > 0.0f) & (x6 > 0.5f);
     const float dlj = flip ? eij * dbase : -eij * dbase;
     const float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
