@@ -108,6 +108,25 @@ def cpu_baseline(wl, nsteps, threads, gpu_records):
     }
 
 
+def energy_check(e, wl, replicas):
+    """Full energies of the device state after the run: the device's full
+    evaluation and its tracked (sum of accepted dE) energy against the
+    oracle's FP64 full energy of the same conformations."""
+    from oracle import pyoracle
+
+    sa = wl.systems[0]
+    osys = pyoracle.OracleSystem(sa, wl.ff)
+    full = e.energy_all()
+    tracked = e.energies()
+    temps = e.temperatures()
+    rel, trk = 0.0, 0.0
+    for r in replicas:
+        want = osys.energy(e.state(r)["pos"])
+        rel = max(rel, abs(full[r] - want) / max(abs(want), 1e-12))
+        trk = max(trk, abs(tracked[r] - want) / ffm.kT(temps[r]))
+    return {"replicas": list(replicas), "max_rel_E_err_full": rel, "max_abs_tracked_E_err_kT": trk}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -259,6 +278,8 @@ def main():
         threads = args.cpu_threads or min(R, os.cpu_count() or 1)
         cpu = cpu_baseline(wl, min(args.cpu_steps, rec_n), threads, gpu_records)
         log("cpu: %.2f s" % cpu["seconds"])
+        cpu["energy_check"] = energy_check(e, wl, range(0, R, max(1, R // 4)))
+        log("energy check:", cpu["energy_check"])
 
     out = {
         "metric": METRIC,
