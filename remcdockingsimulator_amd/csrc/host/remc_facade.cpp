@@ -306,6 +306,22 @@ bool Molecule::rotateQ(Vector3double axis, double angle)
     return true;
 }
 
+void Molecule::saveBeforeStateChange(const Molecule *m)
+{
+    Residues = m->Residues;
+    center = m->center;
+    position = m->position;
+    rotation = m->rotation;
+}
+
+void Molecule::undoStateChange(const Molecule *m)
+{
+    Residues = m->Residues;
+    center = m->center;
+    position = m->position;
+    rotation = m->rotation;
+}
+
 void Molecule::setMoleculeRoleIdentifier(float role)
 {
     // Molecule.cpp:118-139.  The reference also overwrites crowder residues'
@@ -486,6 +502,148 @@ double Replica::E(int device)
         ok(remc_energy_all(ctx.c, &e), "remc_energy_all"))
         potential = e;
     return e;
+}
+
+void Replica::DeviceMirror::reset()
+{
+    if (ctx) remc_ctx_destroy(ctx);
+    ctx = nullptr;
+    sys = -1;
+    seen = remc_counters{};
+}
+
+void Replica::copy(const Replica &r)
+{
+    label = r.label;
+    temperature = r.temperature;
+    aminoAcids = r.aminoAcids;
+    boundingValue = r.boundingValue;
+    nonCrowderCount = r.nonCrowderCount;
+    molecules = r.molecules;
+    potential = r.potential;
+    blockSize = r.blockSize;
+}
+
+void Replica::initRNGs(uint64_t seed) { rngSeed = seed; }
+
+int Replica::ReplicaDataToDevice(int device)
+{
+    mirror.reset();
+    remc_ctx *c = nullptr;
+    if (!ok(remc_ctx_create(device, &c), "remc_ctx_create")) return -1;
+    mirror.ctx = c;
+    remc_forcefield ff = makeForceField(aminoAcids);
+    Packed p = pack();
+    remc_system_desc d = p.desc();
+    d.box = boundingValue;
+    double T = temperature;
+    float ts = translateStep;
+    double rs = rotateStep;
+    int first = -1;
+    // one stream family per (seed, label): the label spreads replicas of one seed apart
+    const uint64_t key = rngSeed ^ (0x9E3779B97F4A7C15ull * (uint64_t)(label + 1));
+    if (!ok(remc_set_forcefield(c, &ff), "remc_set_forcefield") || !ok(remc_set_seed(c, key), "remc_set_seed") ||
+        !ok(remc_system_add(c, &d, &mirror.sys), "remc_system_add") ||
+        !ok(remc_replicas_add(c, mirror.sys, 1, &T, &ts, &rs, &first), "remc_replicas_add")) {
+        mirror.reset();
+        return -1;
+    }
+    return 0;
+}
+
+int Replica::MoleculeDataToDevice(int moleculeId)
+{
+    if (moleculeId < 0 || moleculeId >= (int)molecules.size()) {
+        std::fprintf(stderr, "MoleculeDataToDevice: molecule %d out of range\n", moleculeId);
+        return -1;
+    }
+    if (!mirror.ctx) return ReplicaDataToDevice();
+    // the whole conformation goes up (one copy per array); the tracked
+    // energy is then re-evaluated
+    Packed p = pack();
+    if (!ok(remc_set_state(mirror.ctx, 0, p.x.data(), p.y.data(), p.z.data(), p.rx.data(), p.ry.data(), p.rz.data(),
+                           p.cx.data(), p.cy.data(), p.cz.data(), p.quat.data()),
+            "remc_set_state") ||
+        !ok(remc_refresh_energies(mirror.ctx), "remc_refresh_energies"))
+        return -1;
+    return 0;
+}
+
+void Replica::FreeDevice() { mirror.reset(); }
+
+double Replica::EonDevice()
+{
+    if (molecules.empty()) return 0.0;
+    if (!mirror.ctx ? ReplicaDataToDevice() != 0 : MoleculeDataToDevice(0) != 0) return NAN;
+    double e = NAN;
+    if (!ok(remc_energy_plain(mirror.ctx, 0, blockSize, &e), "remc_energy_plain")) return NAN;
+    return e;
+}
+
+int Replica::MCSearch(int steps)
+{
+    if (steps <= 0 || molecules.empty()) return 0;
+    if (!mirror.ctx ? ReplicaDataToDevice() != 0 : MoleculeDataToDevice(0) != 0) return -1;
+    remc_counters before{};
+    if (!ok(remc_get_counters(mirror.ctx, &before), "remc_get_counters") ||
+        !ok(remc_mc_steps(mirror.ctx, steps), "remc_mc_steps"))
+        return -1;
+    Packed p = pack();
+    remc_counters after{};
+    double e = 0.0;
+    // potential is the full energy of the final conformation, as the reference's
+    // MCSearch leaves it (the device tracks the sum of accepted dE, whose FP32
+    // pair-term rounding accumulates over accepted steps)
+    if (!ok(remc_get_state(mirror.ctx, 0, p.x.data(), p.y.data(), p.z.data(), p.rx.data(), p.ry.data(), p.rz.data(),
+                           p.cx.data(), p.cy.data(), p.cz.data(), p.quat.data()),
+            "remc_get_state") ||
+        !ok(remc_refresh_energies(mirror.ctx), "remc_refresh_energies") ||
+        !ok(remc_get_energies(mirror.ctx, &e), "remc_get_energies") ||
+        !ok(remc_get_counters(mirror.ctx, &after), "remc_get_counters"))
+        return -1;
+    size_t k = 0;
+    for (size_t m = 0; m < molecules.size(); ++m) {
+        Molecule &mol = molecules[m];
+        for (Residue &r : mol.Residues) {
+            r.position = Vector3f(p.x[k], p.y[k], p.z[k]);
+            r.relativePosition = Vector3f(p.rx[k], p.ry[k], p.rz[k]);
+            ++k;
+        }
+        mol.center = Vector3f(p.cx[m], p.cy[m], p.cz[m]);
+        mol.position = mol.center;
+        mol.rotation = Quaternion(p.quat[4 * m], p.quat[4 * m + 1], p.quat[4 * m + 2], p.quat[4 * m + 3]);
+    }
+    potential = e;
+    accept += (long)(after.accept - before.accept);
+    acceptA += (long)(after.accept_boltz - before.accept_boltz);
+    reject += (long)(after.reject - before.reject);
+    return 0;
+}
+
+bool Replica::sample(FILE *boundConformations, int current_step, float boundEnergyThreshold, bool recordAllSamples)
+{
+    samplesSinceLastExchange++;
+    float ncp = (float)potential;
+    if ((int)molecules.size() != nonCrowderCount) {
+        // the pairs between non-crowder molecules, on the device mirror
+        if (!mirror.ctx ? ReplicaDataToDevice() != 0 : MoleculeDataToDevice(0) != 0) return false;
+        double nc = 0.0;
+        if (!ok(remc_energy_noncrowder_all(mirror.ctx, &nc), "remc_energy_noncrowder_all")) return false;
+        ncp += (float)nc;
+    }
+    const bool bound = ncp < boundEnergyThreshold;
+    if (bound) boundSamples++;
+    if (boundConformations && (bound || recordAllSamples)) {
+        std::fprintf(boundConformations, "%d; %0.5f (%0.5f); %0.1f\n", current_step, (double)ncp, potential,
+                     (double)temperature);
+        for (int a = 0; a < (int)molecules.size(); ++a) {
+            const Molecule &m = molecules[a];
+            std::fprintf(boundConformations, "%d %2d: %f %f %f %f %f %f %f\n", a < nonCrowderCount ? 0 : 1, a,
+                         m.rotation.w, m.rotation.x, m.rotation.y, m.rotation.z, (double)m.center.x,
+                         (double)m.center.y, (double)m.center.z);
+        }
+    }
+    return bound;
 }
 
 void Replica::saveAsSinglePDB(const char *filename) const
@@ -1280,7 +1438,11 @@ int REMCSimulation(Replica *initial, argdata *p, std::vector<Replica> *final_rep
         mcstep += mcstepsPerRE;
         if (round >= p->REsteps) continue;
 
-        // exchange step `round` (main.cpp:1378-1469)
+        // exchange step `round` (main.cpp:1378-1469).  The tracked energies are
+        // re-evaluated first: the reference's potential is a full evaluation of
+        // the current conformation, and the running sum of accepted dE would
+        // otherwise carry the FP32 rounding of every accepted step.
+        if (!ok(remc_refresh_energies(ctx.c), "remc_refresh_energies")) return 1;
         if (!ok(remc_get_counters(ctx.c, now.data()), "remc_get_counters")) return 1;
         if (!ok(remc_get_ladder(ctx.c, sys, ladder.data()), "remc_get_ladder")) return 1;
         if (p->MCsteps / p->REsteps * (round - 1) >= p->sampleStartsAfter) {

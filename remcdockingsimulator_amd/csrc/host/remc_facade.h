@@ -91,6 +91,10 @@ class Molecule {
     bool rotateQ(Vector3double axis, double angle);
     void setRotation(Quaternion q);
     void setMoleculeRoleIdentifier(float role);
+    // Molecule.cpp:88-102: residues, centre, position and rotation copied from
+    // m (the state before a move), and copied back from m to undo the move
+    void saveBeforeStateChange(const Molecule *m);
+    void undoStateChange(const Molecule *m);
     bool isCrowder() const { return moleculeRoleIdentifier == CROWDER_IDENTIFIER; }
     size_t residueCount() const { return Residues.size(); }
 };
@@ -158,6 +162,23 @@ bool loadSimulation(const std::string &path, argdata *p, Checkpoint *ck);
 
 class Replica {
   public:
+    // The replica's own device context (ReplicaDataToDevice ... FreeDevice).
+    // A copy of a Replica does not share it: the copy starts without one.
+    struct DeviceMirror {
+        remc_ctx *ctx = nullptr;
+        int sys = -1;
+        remc_counters seen{};  // counters already folded into accept/acceptA/reject
+        DeviceMirror() = default;
+        DeviceMirror(const DeviceMirror &) {}
+        DeviceMirror &operator=(const DeviceMirror &o)
+        {
+            if (this != &o) reset();
+            return *this;
+        }
+        ~DeviceMirror() { reset(); }
+        void reset();
+    };
+
     std::vector<Molecule> molecules;
     AminoAcids aminoAcids;
     float temperature = 300.0f;
@@ -170,6 +191,9 @@ class Replica {
     // statistics (Replica.h:119-131)
     long accept = 0, acceptA = 0, reject = 0, totalAccept = 0, totalAcceptReject = 0;
     long boundSamples = 0, samplesSinceLastExchange = 0, totalBoundSamples = 0, totalSamples = 0;
+    int blockSize = 0;      // tile of the plain kernel (0: 32 below 1024 beads, else 64)
+    uint64_t rngSeed = 0;   // key of the replica's counter-based RNG streams (initRNGs)
+    DeviceMirror mirror;
 
     void setAminoAcidData(const AminoAcids &a) { aminoAcids = a; }
     int loadMolecule(const char *pdbfilename);
@@ -184,6 +208,38 @@ class Replica {
     void exchangeReplicas(Replica &r); // Replica.cpp:145-159
     // Full energy on the GPU (Replica::E / EonDevice); a single-replica context.
     double E(int device = 0);
+
+    // Replica.cpp:161-216: label, temperature, amino acids, box, molecules,
+    // potential and block size of r (not the device context)
+    void copy(const Replica &r);
+    // Replica.cpp:265-290.  The reference seeds seven GSL streams from the
+    // clock; here one counter-based (Philox) stream family keyed by seed and
+    // label, taken by the device context ReplicaDataToDevice creates.
+    void initRNGs(uint64_t seed = 0);
+    // Replica.cpp:1108-1121: the plain kernel's tile (32, 64, 128 or 256)
+    void setBlockSize(int b) { blockSize = b; }
+    // Replica.cpp:1308-1417: this replica's system and state on a device
+    // context of its own; returns 0, or -1 with the library's message printed
+    int ReplicaDataToDevice(int device = 0);
+    // Replica.cpp:1530-1576: the host conformation (after host-side moves of
+    // molecule moleculeId, or of any molecule) to the device mirror
+    int MoleculeDataToDevice(int moleculeId);
+    // Replica.cpp:1618-1655
+    void FreeDevice();
+    // Replica.cpp:1601-1616: the reference-shaped tiled kernel on the device
+    // mirror of the host conformation (created on first use)
+    double EonDevice();
+    // Replica.cpp:368-478: steps MC steps on the device mirror (created on
+    // first use) from the host conformation; the conformation, potential and
+    // the accept / acceptA / reject counters come back.  Returns 0 or -1.
+    int MCSearch(int steps);
+    // Replica.cpp:1717-1780: counts the sample; E_complex = potential plus the
+    // energy between non-crowder molecules (on the device mirror, accumulated
+    // in float as the reference does); if E_complex < boundEnergyThreshold
+    // (or recordAllSamples) appends the boundconformations record.  Returns
+    // whether the sample is bound.
+    bool sample(FILE *boundConformations, int current_step, float boundEnergyThreshold,
+                bool recordAllSamples = false);
     void saveAsSinglePDB(const char *filename) const; // Replica.cpp:495-530
     // one MODEL of a multi-model trajectory ("post processor/postprocess/main.cpp:252-291")
     void writeToTrajectory(FILE *file, int frame, double drms) const;

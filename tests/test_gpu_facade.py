@@ -7,6 +7,7 @@ bound-conformation record's poses are rebuilt from the printed quaternions
 and centres and its printed energies recomputed in FP64."""
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -162,3 +163,64 @@ def test_remc_sim_checkpoint_resume_is_exact(tmp_path, aas):
     bb = open(pb + "boundconformations").read().splitlines()
     cut = next((i for i, x in enumerate(ba) if re.match(r"^\d+;", x) and int(x.split(";")[0]) >= 1000), len(ba))
     assert bb == ba[cut:]
+
+
+def test_replica_device_api(tmp_path, oracle):
+    """The reference's per-replica device calls through the facade
+    (Replica::ReplicaDataToDevice, EonDevice, MCSearch, MoleculeDataToDevice,
+    sample, FreeDevice; Replica.cpp:368-478,1308-1417,1530-1655,1717-1780):
+    the plain kernel and the full energy agree; MCSearch takes the oracle's
+    steps from the same counters (seed 1234 mixed with label 3) and brings
+    back the state, potential and counters; a host move reaches the device;
+    sample writes the reference's record."""
+    from facade_api_harness import DATA as FDATA
+    from facade_api_harness import build, read_dump
+
+    tmp = str(tmp_path)
+    exe = build(tmp)
+    paths = split_chains(1, tmp)
+    say("facade replica device api")
+    r = subprocess.run([exe, "gpu", FDATA, paths["A"], paths["B"], tmp], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = {ln.split()[0]: ln.split()[1:] for ln in r.stdout.strip().split("\n")}
+    e_plain, e_full = (float.fromhex(v) for v in out["energy"])
+    assert abs(e_plain - e_full) <= 1e-5 * abs(e_full)
+    pot, acc, acc_a, rej = float.fromhex(out["search"][0]), *(int(v) for v in out["search"][1:])
+    assert acc + acc_a + rej == 600
+    after = float.fromhex(out["after"][0])
+    kt = ffm.kT(320.0)
+    assert abs(pot - after) <= 1e-5 * abs(after) + 1e-4 * kt
+    moved_plain, moved_full = (float.fromhex(v) for v in out["moved"])
+    assert abs(moved_plain - moved_full) <= 1e-5 * abs(moved_full) and moved_full != after
+
+    # the oracle from the same initial conformation and counters
+    d0, d1 = read_dump(os.path.join(tmp, "initial.txt")), read_dump(os.path.join(tmp, "searched.txt"))
+    n = len(d0["x"])
+    body = np.repeat(np.arange(len(d0["crowder"])), np.diff(d0["body_start"]))
+    sa = model.SystemArrays(
+        body_start=np.array(d0["body_start"], np.int32), type=np.array(d0["type"], np.int32),
+        charge=np.array(d0["charge"], np.float32), sigma=np.array(d0["sigma"], np.float32),
+        crowder=np.array(d0["crowder"], np.uint8), box=d0["box"][0],
+        pos=np.stack([d0["x"], d0["y"], d0["z"]], 1).astype(np.float32),
+        rel=np.stack([d0["rx"], d0["ry"], d0["rz"]], 1).astype(np.float32),
+        center=np.stack([d0["cx"], d0["cy"], d0["cz"]], 1).astype(np.float32),
+        quat=np.array(d0["quat"]).reshape(-1, 4))
+    assert body.shape[0] == n
+    ff = ffm.ForceField.mj1996(DATA)
+    o = oracle.OracleSystem(sa, ff)
+    key = (1234 ^ (0x9E3779B97F4A7C15 * 4)) & 0xFFFFFFFFFFFFFFFF
+    rep = oracle.OracleReplica(o, 320.0, 2.0, 0.2, 0)
+    rec = rep.mc(seed=key, nsteps=600, mode=0)
+    pos1 = np.stack([d1["x"], d1["y"], d1["z"]], 1).astype(np.float32)
+    if np.sum(rec["accepted"] == 0) == rej and np.array_equal(pos1, rep.pos):
+        assert np.array_equal(np.array(d1["quat"]).reshape(-1, 4), rep.quat)
+    else:
+        assert np.min(rec["margin"]) < 1e-4  # a near-threshold decision
+    assert abs(pot - o.energy(pos1)) <= 1e-5 * abs(pot) + 1e-4 * kt
+
+    with open(os.path.join(tmp, "bound.txt")) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    assert re.match(r"^77; -?[\d.]+ \(-?[\d.]+\); 320\.0$", lines[0]), lines[0]
+    assert [ln.split()[:2] for ln in lines[1:4]] == [["0", "0:"], ["0", "1:"], ["1", "2:"]]
+    assert len(lines) == 4  # the unbound sample wrote nothing
+    assert out["sample"] == ["1", "0", "1", "2"] and out["freed"] == ["1"]
