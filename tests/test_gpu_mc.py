@@ -291,3 +291,40 @@ def test_single_body_and_all_crowder_systems(oracle):
         orec = orep.mc(seed=5, nsteps=200, mode=0)
         compare_decisions(e.records(2 + r), orec, [300.0, 320.0, 340.0][r])
     e.close()
+
+
+def test_mc_uneven_systems_in_one_launch(oracle):
+    """Config 3: 64 systems of uneven bead, body and crowder counts (8 replicas
+    each) in one persistent launch; replicas of several systems, from the
+    first to the last, follow the oracle step for step."""
+    wl = synth.config(3)
+    nsteps = 200
+    e = engine_for(wl, seed=SEED)
+    e.set_recording(nsteps)
+    e.mc_steps(nsteps)
+    e.synchronize()
+    energies = e.energies()
+    full = e.energy_all()
+    diverged = 0
+    first = np.cumsum([0] + [len(t) for t in wl.temperatures])
+    for s in (0, 17, 40, 63):
+        sa = wl.systems[s]
+        o = oracle.OracleSystem(sa, wl.ff)
+        for k in (0, 7):
+            r = int(first[s] + k)
+            T = wl.temperatures[s][k]
+            say("config 3 system", s, "beads", sa.n_beads, "replica", r)
+            orep = oracle.OracleReplica(o, T, wl.translate_step, wl.rotate_step, r)
+            orec = orep.mc(seed=SEED, nsteps=nsteps, mode=0)
+            n_eq, div = compare_decisions(e.records(r), orec, T)
+            st = e.state(r)
+            if not div:
+                assert np.array_equal(st["pos"], orep.pos), r
+                assert np.array_equal(st["quat"], orep.quat), r
+                assert abs(energies[r] - orep.energy) < 1e-4 * kT(T) + 1e-5 * abs(orep.energy)
+            else:
+                diverged += 1
+            want = o.energy(st["pos"])
+            assert abs(full[r] - want) <= 1e-5 * abs(want) + 1e-7
+    e.close()
+    assert diverged <= 1
