@@ -238,6 +238,31 @@ def test_exchange_matches_oracle(oracle):
     e.close()
 
 
+def test_exchange_every_system_ladder(oracle):
+    """Config 3: one exchange call runs the rounds of all 64 temperature
+    ladders (one per system); each ladder's swaps equal the oracle's round
+    for that system (the slot-replica map holds global replica indices)."""
+    wl = synth.config(3)
+    e = engine_for(wl, seed=SEED)
+    first = np.cumsum([0] + [len(t) for t in wl.temperatures])
+    slots = [np.arange(len(t)) + first[s] for s, t in enumerate(wl.temperatures)]
+    for rnd in range(6):
+        e.mc_steps(20)
+        en = e.energies()
+        n = e.exchange()
+        total = 0
+        for s, lad in enumerate(wl.temperatures):
+            loc = slots[s] - first[s]
+            want, nw, _ = oracle.exchange_round(lad, loc, en[first[s]:first[s + 1]], rnd % 2, rnd, s, SEED)
+            total += nw
+            got = e.ladder(s)
+            assert np.array_equal(got, want + first[s]), (rnd, s, got, want)
+            slots[s] = want + first[s]
+        say("exchange round", rnd, "swaps", n)
+        assert n == total
+    e.close()
+
+
 def test_graph_run_equals_eager(oracle):
     """remc_run (hipGraph replay of MC + exchange rounds) gives bit-identical
     state to the same calls made eagerly, over several replays."""
