@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--coop", type=int, default=0, help="MC workgroups per replica (0 = autotune)")
     ap.add_argument("--fb-steps", type=int, default=2000,
                     help="force-biased MC steps per replica timed after the main run (0 = skip)")
+    ap.add_argument("--adjoint-calls", type=int, default=20,
+                    help="adjoint (gradient) calls timed after the main run (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -272,6 +274,26 @@ def main():
                                "2 n_m (N - n_m) pair energies and 2 n_m (N - n_m) pair derivatives"}
         log("fb: %.3f s for %d steps" % (fw, args.fb_steps))
 
+    # -------------------------------------------------------- adjoint ----
+    adj = None
+    if args.adjoint_calls > 0 and len(wl.systems) == 1:
+        e.adjoint(0)  # warm
+        e.timing_enable(True)
+        e.synchronize()
+        t6 = time.perf_counter()
+        for k in range(args.adjoint_calls):
+            e.adjoint(k % R)
+        t7 = time.perf_counter()
+        ams, anl = e.timing(Engine.ADJOINT)
+        # every ordered inter-body bead pair once: 2 P_full pair derivatives per call
+        dpairs = 2 * wl.pairs_full()
+        adj = {"calls": args.adjoint_calls, "kernel_avg_ms": ams / max(anl, 1),
+               "pair_derivatives_per_s": dpairs / (ams / max(anl, 1) / 1e3),
+               "calls_per_s_wall": args.adjoint_calls / (t7 - t6),
+               "kernel": "k_adjoint_pairs + k_adjoint_reduce + k_adjoint_bodies (forces, torques, dE/d{lambda,e0,xi} of one replica)",
+               "work_per_call": "2 P_full ordered pair derivatives (P_full = %d)" % wl.pairs_full()}
+        log("adjoint: %.3f ms per call (kernels)" % adj["kernel_avg_ms"])
+
     # ------------------------------------------------------------ cpu ----
     cpu = None
     if not args.no_cpu and len(wl.systems) == 1 and gpu_records is not None:
@@ -306,6 +328,7 @@ def main():
                              "mc_steps": (steps_total / wall) / plain["mc_steps_per_s"]} if plain else None,
         "cpu_baseline": cpu,
         "force_biased": fb,
+        "adjoint": adj,
         "energy_mean": float(np.mean(energies)),
     }
     e.close()

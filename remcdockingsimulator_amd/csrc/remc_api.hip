@@ -1436,19 +1436,18 @@ int remc_adjoint(remc_ctx *c, int replica, double *force_torque, double *dparams
     CHECK_REP(c, replica);
     if (!force_torque && !dparams) return fail(REMC_ERR_ARG, "null outputs");
     const HostSys &s = c->systems[c->reps[replica].sys];
-    const int tiles = (s.n_beads + FULL_TILE - 1) / FULL_TILE;
-    if (!c->d_adj) {
-        const size_t nd = 3 * (size_t)c->max_beads + 3 * (size_t)c->max_tiles + 6 * (size_t)c->max_bodies + 3;
-        HIP_TRY(hipMalloc(&c->d_adj, nd * sizeof(double)));
-    }
-    double *grad = c->d_adj, *part = grad + 3 * (size_t)c->max_beads, *ft = part + 3 * (size_t)c->max_tiles,
-           *dp = ft + 6 * (size_t)c->max_bodies;
+    // scratch: per-chunk bead gradients, bead gradients, per-workgroup parameter
+    // partials, then the per-body outputs and the parameter totals
+    const size_t nd = adjoint_scratch_doubles(c->max_beads, c->max_bodies);
+    if (!c->d_adj) HIP_TRY(hipMalloc(&c->d_adj, nd * sizeof(double)));
+    double *ft = c->d_adj + nd - 6 * (size_t)c->max_bodies - 3, *dp = ft + 6 * (size_t)c->max_bodies;
     std::vector<unsigned char> af(adjoint_ff_size());
     adjoint_ff_fill(af.data(), c->ff.table, c->ff.lambda, c->ff.e0, c->ff.xi, c->ff.lj_cf * c->ff.kbt_cf,
                     c->ff.dh_cf * c->ff.kbt_cf);
     {
         TimedLaunch t(c, 4);
-        HIP_TRY(launch_adjoint(c->params(), af.data(), replica, s.n_beads, s.n_bodies, grad, part, ft, dp, c->stream));
+        HIP_TRY(launch_adjoint(c->params(), af.data(), replica, s.n_beads, s.n_bodies, c->max_beads, c->d_adj, ft, dp,
+                               c->stream));
     }
     if (force_torque)
         HIP_TRY(hipMemcpyAsync(force_torque, ft, sizeof(double) * 6 * s.n_bodies, hipMemcpyDeviceToHost, c->stream));

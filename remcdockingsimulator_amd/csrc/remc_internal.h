@@ -109,7 +109,8 @@ hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
 hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
 hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
-                          double *bead_grad, double *param_part, double *force_torque, double *dparams, hipStream_t s);
+                          int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s);
+size_t adjoint_scratch_doubles(int max_beads, int max_bodies);
 hipError_t launch_drms(const KParams &p, int sys_id, int n_beads, int n_rep, const float *rx, const float *ry,
                        const float *rz, double *partials, double inv_pairs, double *out, hipStream_t s);
 hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, int n_types,

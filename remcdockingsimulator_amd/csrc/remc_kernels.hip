@@ -1518,35 +1518,53 @@ hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
 // --------------------------------------------------------------------------
 // adjoint: gradient of the full energy of one replica
 //
-// k_adjoint_beads: one thread per bead i, partner beads staged in 256-bead
-// LDS tiles; FP32 pair derivatives (same branches as the pair term),
-// FP64 accumulation of dE/dx_i and of the parameter derivatives
+// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of
+// ADJ_TJ); thread i of a tile accumulates, over the chunk's partner beads
+// (staged in LDS), the FP32 pair derivatives (same branches as the pair term)
+// into FP64 dE/dx_i, and the parameter derivatives
 // dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij,
-// dE/dxi = sum DH r / xi^2.  k_adjoint_bodies: per body F = -sum dE/dx_i and
-// torque sum rel_i x F_i about the body centre, and the parameter totals.
+// dE/dxi = sum DH r / xi^2 over the unordered pairs it owns.  Partials go to
+// per-chunk slots; k_adjoint_reduce adds a bead's chunks in chunk order and
+// k_adjoint_bodies forms per body F = -sum dE/dx_i, the torque sum rel_i x F_i
+// about the body centre, and the parameter totals, so the result does not
+// depend on scheduling.
 // --------------------------------------------------------------------------
 struct AdjFF {
     float table[REMC_NTYPES * REMC_NTYPES]; // raw table T_ij
     float lambda, e0, xi, c_lj, c_dh;
 };
 
-__global__ __launch_bounds__(FULL_TILE) void k_adjoint_beads(KParams p, AdjFF af, int replica, double *bead_grad,
-                                                             double *param_part)
+constexpr int ADJ_TI = 64;  // beads per tile (one wave)
+constexpr int ADJ_TJ = 128; // partner beads per chunk
+
+__global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, int replica, double *part_grad,
+                                                          double *param_part)
 {
     __shared__ float s_tab[REMC_NTYPES * REMC_NTYPES];
-    __shared__ float s_x[FULL_TILE], s_y[FULL_TILE], s_z[FULL_TILE], s_q[FULL_TILE], s_s[FULL_TILE];
-    __shared__ int s_tc[FULL_TILE], s_body[FULL_TILE];
-    __shared__ double s_red[3][FULL_TILE / 64];
+    __shared__ float s_x[ADJ_TJ], s_y[ADJ_TJ], s_z[ADJ_TJ], s_q[ADJ_TJ], s_s[ADJ_TJ];
+    __shared__ int s_tc[ADJ_TJ], s_body[ADJ_TJ];
     const int tid = threadIdx.x;
     const DevRep R = p.rep[replica];
     const DevSys S = p.sys[R.sys];
     const DevFF ff = p.ff;
     const int N = S.n_beads;
-    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += FULL_TILE) s_tab[k] = af.table[k];
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += ADJ_TI) s_tab[k] = af.table[k];
     const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
     const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
-    const int i = blockIdx.x * FULL_TILE + tid;
+    const int i = blockIdx.x * ADJ_TI + tid;
+    const int jc = blockIdx.y, j0 = jc * ADJ_TJ;
+    const int nj = min(ADJ_TJ, N - j0);
+    for (int k = tid; k < ADJ_TJ; k += ADJ_TI) {
+        const int jt = j0 + k;
+        if (k < nj) {
+            s_x[k] = gx[jt]; s_y[k] = gy[jt]; s_z[k] = gz[jt];
+            s_q[k] = qs[jt]; s_s[k] = sgs[jt]; s_tc[k] = tcs[jt]; s_body[k] = bodies[jt];
+        } else {
+            s_body[k] = -2;
+        }
+    }
+    __syncthreads();
     const bool vi = i < N;
     const float xi = vi ? gx[i] : 0.f, yi = vi ? gy[i] : 0.f, zi = vi ? gz[i] : 0.f;
     const int tci = vi ? tcs[i] : 0, bi = vi ? bodies[i] : -1;
@@ -1554,18 +1572,7 @@ __global__ __launch_bounds__(FULL_TILE) void k_adjoint_beads(KParams p, AdjFF af
     const bool ci = (tci >> 8) != 0;
     double gxa = 0.0, gya = 0.0, gza = 0.0, dl = 0.0, de0 = 0.0, dxi = 0.0;
     const float inv_xi2 = 1.0f / (af.xi * af.xi);
-    for (int j0 = 0; j0 < N; j0 += FULL_TILE) {
-        __syncthreads();
-        const int jt = j0 + tid;
-        if (jt < N) {
-            s_x[tid] = gx[jt]; s_y[tid] = gy[jt]; s_z[tid] = gz[jt];
-            s_q[tid] = qs[jt]; s_s[tid] = sgs[jt]; s_tc[tid] = tcs[jt]; s_body[tid] = bodies[jt];
-        } else {
-            s_body[tid] = -2;
-        }
-        __syncthreads();
-        if (!vi) continue;
-        const int nj = min(FULL_TILE, N - j0);
+    if (vi) {
         for (int k = 0; k < nj; ++k) {
             if (s_body[k] == bi) continue;
             const int tcj = s_tc[k];
@@ -1609,27 +1616,30 @@ __global__ __launch_bounds__(FULL_TILE) void k_adjoint_beads(KParams p, AdjFF af
             gya += (double)(kf * dy);
             gza += (double)(kf * dz);
         }
-    }
-    if (vi) {
-        bead_grad[3 * i + 0] = gxa;
-        bead_grad[3 * i + 1] = gya;
-        bead_grad[3 * i + 2] = gza;
+        double *pg = part_grad + 3 * ((size_t)jc * N + i);
+        pg[0] = gxa;
+        pg[1] = gya;
+        pg[2] = gza;
     }
     const double v0 = wave_sum_dpp(dl), v1 = wave_sum_dpp(de0), v2 = wave_sum_dpp(dxi);
-    if ((tid & 63) == 0) {
-        s_red[0][tid >> 6] = v0;
-        s_red[1][tid >> 6] = v1;
-        s_red[2][tid >> 6] = v2;
-    }
-    __syncthreads();
-    if (tid < 3) {
-        double t = 0.0;
-        for (int w = 0; w < FULL_TILE / 64; ++w) t += s_red[tid][w];
-        param_part[3 * blockIdx.x + tid] = t;
+    if (tid == 0) {
+        double *pp = param_part + 3 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+        pp[0] = v0;
+        pp[1] = v1;
+        pp[2] = v2;
     }
 }
 
-__global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad, const double *param_part, int n_tiles,
+__global__ void k_adjoint_reduce(int n_beads, int n_chunks, const double *part_grad, double *bead_grad)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * n_beads) return;
+    double g = 0.0;
+    for (int c = 0; c < n_chunks; ++c) g += part_grad[(size_t)c * 3 * n_beads + t];
+    bead_grad[t] = g;
+}
+
+__global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad, const double *param_part, int n_parts,
                                  double *force_torque, double *dparams)
 {
     const DevRep R = p.rep[replica];
@@ -1638,7 +1648,7 @@ __global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad
     if (m == 0) {
         for (int k = 0; k < 3; ++k) {
             double t = 0.0;
-            for (int a = 0; a < n_tiles; ++a) t += param_part[3 * a + k];
+            for (int a = 0; a < n_parts; ++a) t += param_part[3 * a + k];
             dparams[k] = t;
         }
     }
@@ -1660,14 +1670,26 @@ __global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad
     }
 }
 
-hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
-                          double *bead_grad, double *param_part, double *force_torque, double *dparams, hipStream_t s)
+size_t adjoint_scratch_doubles(int max_beads, int max_bodies)
 {
-    const int tiles = (n_beads + FULL_TILE - 1) / FULL_TILE;
+    const size_t ti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, tj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
+    return 3 * tj * (size_t)max_beads + 3 * (size_t)max_beads + 3 * ti * tj + 6 * (size_t)max_bodies + 3;
+}
+
+hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
+                          int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s)
+{
+    const int ti = (n_beads + ADJ_TI - 1) / ADJ_TI, tj = (n_beads + ADJ_TJ - 1) / ADJ_TJ;
+    const size_t mti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, mtj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
+    double *part_grad = scratch, *bead_grad = part_grad + 3 * mtj * (size_t)max_beads;
+    double *param_part = bead_grad + 3 * (size_t)max_beads;
+    (void)mti;
     const AdjFF af = *(const AdjFF *)adj_ff;
-    hipLaunchKernelGGL(k_adjoint_beads, dim3(tiles), dim3(FULL_TILE), 0, s, p, af, replica, bead_grad, param_part);
+    hipLaunchKernelGGL(k_adjoint_pairs, dim3(ti, tj), dim3(ADJ_TI), 0, s, p, af, replica, part_grad, param_part);
+    hipLaunchKernelGGL(k_adjoint_reduce, dim3((3 * n_beads + 255) / 256), dim3(256), 0, s, n_beads, tj, part_grad,
+                       bead_grad);
     hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 63) / 64), dim3(64), 0, s, p, replica, bead_grad, param_part,
-                       tiles, force_torque, dparams);
+                       ti * tj, force_torque, dparams);
     return hipGetLastError();
 }
 
