@@ -121,9 +121,11 @@ __device__ __forceinline__ int div_ceil_small(int a, int b) { return div_floor_s
 
 // Work split of one step's (moved bead x partner bead) pairs over a
 // workgroup: groups of G lanes, lane l of group g takes partners l, l+G, ...
-// (t = ceil(nj/G) of them) against moved beads [g*kper, (g+1)*kper).  Of the
-// splits with t = ceil(nj/BLOCK) .. +3 j-iterations per lane, the one with the
-// fewest pairs per thread (t * kper) is taken.
+// (t = ceil(nj/G) of them, held in registers two at a time) against moved
+// beads [g*kper, (g+1)*kper).  A moved bead's staging is read from LDS once
+// per pair of partners, and that LDS traffic weighs as much as the pair
+// arithmetic, so a split costs kper * (t + 2 ceil(t/2)); of the splits with
+// t = ceil(nj/BLOCK) .. +3 partner iterations per lane the cheapest is taken.
 template <int BLOCK>
 __device__ __forceinline__ void pick_split(int nm, int nj, int &G, int &kper)
 {
@@ -138,8 +140,9 @@ __device__ __forceinline__ void pick_split(int nm, int nj, int &G, int &kper)
         const int Gt = div_ceil_small(nj, t);
         const int groups = div_floor_small(BLOCK, Gt);
         const int kp = div_ceil_small(nm, groups);
-        if (t * kp < best) {
-            best = t * kp;
+        const int cost = kp * (t + 2 * ((t + 1) >> 1));
+        if (cost < best) {
+            best = cost;
             G = Gt;
             kper = kp;
         }
@@ -724,44 +727,54 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         float acc = 0.0f, comp = 0.0f;
-        for (int jj = l; jj < nj; jj += G) {
-            const int j = js + jj + (js + jj >= ovs ? ov : 0);
-            float xj, yj, zj, qj, sj;
-            int tcj;
-            if (CACHE) {
-                const int kj = j - clo;
-                xj = s_jx[kj];
-                yj = s_jy[kj];
-                zj = s_jz[kj];
-                qj = s_jq[kj];
-                sj = s_js[kj];
-                tcj = s_jtc[kj];
-            } else {
-                xj = gx[j];
-                yj = gy[j];
-                zj = gz[j];
-                qj = qs[j];
-                sj = sgs[j];
-                tcj = tcs[j];
-            }
-            const bool cj = (tcj >> 8) != 0;
-            if (crow_m || cj) {
-                for (int k = k0; k < k1; ++k) {
-                    const float eo = pair_crowder(dist2(xj, yj, zj, s_old[k], S), ff);
-                    const float en = pair_crowder(dist2(xj, yj, zj, s_new[k], S), ff);
-                    kahan_add(acc, comp, en - eo);
+        // two partner beads in registers per pass (jj0 and jj0 + G): each moved
+        // bead's staging is read once for both
+        for (int jj0 = l; jj0 < nj; jj0 += 2 * G) {
+            float xj[2], yj[2], zj[2], qj[2], sj[2];
+            int tcj[2];
+            bool on[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const int jj = jj0 + u * G;
+                on[u] = jj < nj;
+                const int ju = on[u] ? jj : jj0; // an index that exists, for the partner that is off
+                const int j = js + ju + (js + ju >= ovs ? ov : 0);
+                if (CACHE) {
+                    const int kj = j - clo;
+                    xj[u] = s_jx[kj];
+                    yj[u] = s_jy[kj];
+                    zj[u] = s_jz[kj];
+                    qj[u] = s_jq[kj];
+                    sj[u] = s_js[kj];
+                    tcj[u] = s_jtc[kj];
+                } else {
+                    xj[u] = gx[j];
+                    yj[u] = gy[j];
+                    zj[u] = gz[j];
+                    qj[u] = qs[j];
+                    sj[u] = sgs[j];
+                    tcj[u] = tcs[j];
                 }
-            } else {
-                const float *erow = s_etab + (tcj & 0xff);
-                for (int k = k0; k < k1; ++k) {
-                    const float4 o = s_old[k], n = s_new[k];
-                    const int ti = s_tc[k] & 0xff;
-                    const float eij = erow[ti * REMC_NTYPES];
-                    const float sij = 0.5f * (n.w + sj);
-                    const float sij2 = sij * sij;
-                    const float qq = o.w * qj;
-                    const float eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
-                    const float en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
+            }
+            const bool cr0 = crow_m || (tcj[0] >> 8) != 0, cr1 = crow_m || (tcj[1] >> 8) != 0;
+            for (int k = k0; k < k1; ++k) {
+                const float4 o = s_old[k], n = s_new[k];
+                const int ti = s_tc[k] & 0xff;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    if (!on[u]) continue;
+                    float eo, en;
+                    if (u == 0 ? cr0 : cr1) {
+                        eo = pair_crowder(dist2(xj[u], yj[u], zj[u], o, S), ff);
+                        en = pair_crowder(dist2(xj[u], yj[u], zj[u], n, S), ff);
+                    } else {
+                        const float eij = s_etab[ti * REMC_NTYPES + (tcj[u] & 0xff)];
+                        const float sij = 0.5f * (n.w + sj[u]);
+                        const float sij2 = sij * sij;
+                        const float qq = o.w * qj[u];
+                        eo = pair_lj_dh(dist2(xj[u], yj[u], zj[u], o, S), eij, sij2, qq, ff);
+                        en = pair_lj_dh(dist2(xj[u], yj[u], zj[u], n, S), eij, sij2, qq, ff);
+                    }
                     kahan_add(acc, comp, en - eo);
                 }
             }
