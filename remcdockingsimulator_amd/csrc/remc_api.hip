@@ -191,18 +191,25 @@ struct TimedLaunch {
     hipEvent_t a = nullptr, b = nullptr;
     TimedLaunch(remc_ctx *c_, int cls_) : c(c_), cls(cls_)
     {
-        if (c->timing) {
-            a = get_event(c);
-            b = get_event(c);
-            if (a && b) hipEventRecord(a, c->stream);
-        }
+        if (!c->timing) return;
+        a = get_event(c);
+        b = get_event(c);
+        // a launch that cannot be timed is still made; its events go back to the pool
+        if (!a || !b || hipEventRecord(a, c->stream) != hipSuccess) release();
     }
     ~TimedLaunch()
     {
-        if (a && b) {
-            hipEventRecord(b, c->stream);
+        if (!a) return;
+        if (hipEventRecord(b, c->stream) == hipSuccess)
             c->pending.push_back({cls, a, b});
-        }
+        else
+            release();
+    }
+    void release()
+    {
+        if (a) c->free_events.push_back(a);
+        if (b) c->free_events.push_back(b);
+        a = b = nullptr;
     }
 };
 
@@ -223,7 +230,7 @@ int collect_timing(remc_ctx *c)
 
 void destroy_graph(remc_ctx *c)
 {
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
     c->graph_steps = -1;
 }
@@ -640,9 +647,9 @@ int autotune_mc(remc_ctx *c)
         }
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (auto &b : bufs) hipFree(b.save);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    for (auto &b : bufs) (void)hipFree(b.save);
     c->synced_copies = 1;
     c->mc_block = best_block;
     c->mc_coop = best_coop;
@@ -689,8 +696,9 @@ int remc_ctx_create(int device, remc_ctx **out)
 int remc_ctx_destroy(remc_ctx *c)
 {
     if (!c) return REMC_OK;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    // teardown: errors here leave nothing to recover, the memory goes either way
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     collect_timing(c);
     destroy_graph(c);
     void *ptrs[] = {c->d_tc, c->d_body, c->d_bstart, c->d_bend, c->d_bcrow, c->d_q, c->d_sig, c->d_etab,
@@ -700,9 +708,9 @@ int remc_ctx_destroy(remc_ctx *c)
                     c->d_out, c->d_plain_partials, c->d_rec, c->d_rec_start, c->d_slots, c->d_err, c->d_adj,
                     c->d_ana};
     for (void *p : ptrs)
-        if (p) hipFree(p);
-    for (auto e : c->free_events) hipEventDestroy(e);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) (void)hipFree(p);
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return REMC_OK;
 }
@@ -842,7 +850,7 @@ int remc_energy_plain(remc_ctx *c, int replica, int block, double *out)
     const int G = (s.n_beads + block - 1) / block;
     const size_t need = (size_t)G * G;
     if (need > c->plain_partials_cap) {
-        if (c->d_plain_partials) hipFree(c->d_plain_partials);
+        if (c->d_plain_partials) (void)hipFree(c->d_plain_partials);
         c->d_plain_partials = nullptr;
         HIP_TRY(hipMalloc(&c->d_plain_partials, need * sizeof(float)));
         c->plain_partials_cap = need;
@@ -1094,11 +1102,11 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
         hipError_t e2 = launch_exchange(p, (int)c->systems.size(), c->stream);
         hipError_t e3 = hipStreamEndCapture(c->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return fail(REMC_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e1 != hipSuccess ? e1 : e2 != hipSuccess ? e2 : e3));
         }
         hipError_t e4 = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         if (e4 != hipSuccess) {
             c->graph = nullptr;
             return fail(REMC_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e4));
@@ -1118,7 +1126,7 @@ int remc_set_recording(remc_ctx *c, int max_steps)
     FINALIZE(c);
     if (max_steps < 0) return fail(REMC_ERR_ARG, "negative recording length");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->d_rec) hipFree(c->d_rec);
+    if (c->d_rec) (void)hipFree(c->d_rec);
     c->d_rec = nullptr;
     c->rec_max = 0;
     destroy_graph(c);
@@ -1357,7 +1365,7 @@ static int ana_reserve(remc_ctx *c, size_t bytes)
 {
     if (c->ana_bytes >= bytes) return REMC_OK;
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->d_ana) hipFree(c->d_ana);
+    if (c->d_ana) (void)hipFree(c->d_ana);
     c->d_ana = nullptr;
     c->ana_bytes = 0;
     HIP_TRY(hipMalloc(&c->d_ana, bytes));
