@@ -944,10 +944,14 @@ int remc_mc_steps_fb(remc_ctx *c, int nsteps, int block)
     if (rc != REMC_OK) return rc;
     KParams p = c->params();
     p.lds_mb = std::max(1, c->max_body_beads);
-    const size_t lds = mc_fb_dynamic_lds(p);
-    if ((long long)lds + 8 * 1024 > c->lds_per_block)
+    p.lds_cl = 0;
+    const long long static_lds = 12 * 1024; // table, reductions, 64 batched draws
+    if ((long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block)
         return fail(REMC_ERR_LIMIT, "a body of %d beads does not fit the force-biased kernel's LDS staging",
                     c->max_body_beads);
+    // the replica's partner beads in LDS when they fit (else read from HBM)
+    p.lds_cl = c->max_beads;
+    if (!c->lds_cache || (long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block) p.lds_cl = 0;
     {
         TimedLaunch t(c, 0);
         HIP_TRY(launch_mc_fb(p, (int)c->reps.size(), nsteps, block, c->stream));

@@ -934,7 +934,9 @@ static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, 
 // --------------------------------------------------------------------------
 // force-biased (smart) MC, Rossky, Doll & Friedman (1978), for rigid bodies
 //
-// One workgroup per replica.  Each step: draws (every thread, identical);
+// One workgroup per replica.  The replica's partner beads are cached in LDS
+// when they fit (positions kept current on commit), and wave 0 forms the
+// draws of 64 steps at a time (one step per lane).  Each step: draws;
 // force and torque on the chosen body at its old position (pass A); the
 // proposal d = beta A F + sqrt(2A) n for a translation, the rotation vector
 // theta = beta A tau + sqrt(2A) n for a rotation (thread 0); staging of the
@@ -983,12 +985,18 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
     __shared__ float s_cen[3];
     __shared__ int s_dec;
     __shared__ double s_dE;
+    __shared__ remc_fb_draws s_fbd[64];
     extern __shared__ float4 smem[];
     float4 *s_old = smem;                // old position, q_i c_dh
     float4 *s_orel = s_old + p.lds_mb;   // old relative position, sigma
     float4 *s_new = s_orel + p.lds_mb;   // new position
     float4 *s_nrel = s_new + p.lds_mb;   // new relative position
     int *s_tc = (int *)(s_nrel + p.lds_mb);
+    // partner cache (p.lds_cl beads: the whole replica, or 0 = read from HBM)
+    const bool cached = p.lds_cl > 0;
+    float *s_jx = (float *)(s_tc + p.lds_mb), *s_jy = s_jx + p.lds_cl, *s_jz = s_jy + p.lds_cl;
+    float *s_jq = s_jz + p.lds_cl, *s_js = s_jq + p.lds_cl;
+    int *s_jtc = (int *)(s_js + p.lds_cl);
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r = blockIdx.x;
@@ -1009,6 +1017,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
     const int *tcs = p.tc + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
+    if (cached)
+        for (int b = tid; b < N; b += BLOCK) {
+            s_jx[b] = gx[b];
+            s_jy[b] = gy[b];
+            s_jz[b] = gz[b];
+            s_jq[b] = qs[b];
+            s_js[b] = sgs[b];
+            s_jtc[b] = tcs[b];
+        }
     __syncthreads();
 
     // block sum of v[0..n) into s_ft (FP64, fixed order)
@@ -1028,13 +1045,19 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
 
     for (int it = 0; it < nsteps; ++it, ++step) {
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
-        const remc_fb_draws d = remc_fb_step_draws(step, R.rng_id, p.key0, p.key1, S.n_bodies);
+        if ((it & 63) == 0) {
+            // wave 0 forms the draws of the next 64 steps, one per lane
+            if (wave == 0 && it + lane < nsteps) s_fbd[lane] = remc_fb_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+            __syncthreads();
+        }
+        const remc_fb_draws d = s_fbd[it & 63];
         const int body = d.body;
         const int m0 = p.bstart[S.body_off + body], m1 = p.bend[S.body_off + body], nm = m1 - m0;
         const bool crow_m = p.bcrow[S.body_off + body] != 0;
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
-            s_old[k] = make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
+            s_old[k] = cached ? make_float4(s_jx[b], s_jy[b], s_jz[b], s_jq[b] * ff.c_dh)
+                              : make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
             s_orel[k] = make_float4(grx[b], gry[b], grz[b], sgs[b]);
             s_tc[k] = tcs[b];
         }
@@ -1050,8 +1073,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int jj = l; jj < nj; jj += G) {
             const int j = jj + (jj >= m0 ? nm : 0);
-            const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
-            const int tcj = tcs[j];
+            const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
+            const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
+            const int tcj = cached ? s_jtc[j] : tcs[j];
             const bool crow = crow_m || (tcj >> 8) != 0;
             for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k];
@@ -1117,8 +1141,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         for (int k = 0; k < 6; ++k) v[k] = 0.0;
         for (int jj = l; jj < nj; jj += G) {
             const int j = jj + (jj >= m0 ? nm : 0);
-            const float xj = gx[j], yj = gy[j], zj = gz[j], qj = qs[j], sj = sgs[j];
-            const int tcj = tcs[j];
+            const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
+            const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
+            const int tcj = cached ? s_jtc[j] : tcs[j];
             const bool crow = crow_m || (tcj >> 8) != 0;
             for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k], n = s_new[k], nrl = s_nrel[k];
@@ -1178,6 +1203,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
                 gx[b] = n.x;
                 gy[b] = n.y;
                 gz[b] = n.z;
+                if (cached) {
+                    s_jx[b] = n.x;
+                    s_jy[b] = n.y;
+                    s_jz[b] = n.z;
+                }
                 if (d.rotate) {
                     const float4 nrl = s_nrel[k];
                     grx[b] = nrl.x;
@@ -1225,7 +1255,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
     }
 }
 
-size_t mc_fb_dynamic_lds(const KParams &p) { return (4 * sizeof(float4) + sizeof(int)) * (size_t)p.lds_mb; }
+size_t mc_fb_dynamic_lds(const KParams &p)
+{
+    return (4 * sizeof(float4) + sizeof(int)) * (size_t)p.lds_mb + 24 * (size_t)p.lds_cl;
+}
 
 template <int BLOCK>
 static hipError_t launch_mc_fb_t(const KParams &p, int n_rep, int nsteps, hipStream_t s)
