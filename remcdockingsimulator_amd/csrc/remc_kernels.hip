@@ -1564,8 +1564,9 @@ hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
 // --------------------------------------------------------------------------
 // adjoint: gradient of the full energy of one replica
 //
-// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of
-// ADJ_TJ); thread i of a tile accumulates, over the chunk's partner beads
+// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of tj
+// beads: ADJ_TJ, or N/16 for large systems so that a workgroup has enough
+// work); thread i of a tile accumulates, over the chunk's partner beads
 // (staged in LDS), the FP32 pair derivatives (same branches as the pair term)
 // into FP64 dE/dx_i, and the parameter derivatives
 // dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij,
@@ -1580,15 +1581,24 @@ struct AdjFF {
     float lambda, e0, xi, c_lj, c_dh;
 };
 
-constexpr int ADJ_TI = 64;  // beads per tile (one wave)
-constexpr int ADJ_TJ = 128; // partner beads per chunk
+constexpr int ADJ_TI = 64;       // beads per tile (one wave)
+constexpr int ADJ_TJ = 128;      // smallest partner chunk
+constexpr int ADJ_TJ_MAX = 2048; // largest partner chunk (LDS)
 
-__global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, int replica, double *part_grad,
-                                                          double *param_part)
+__device__ __host__ inline int adjoint_chunk(int n_beads)
+{
+    int c = (n_beads + 15) / 16;
+    c = ((c + 63) / 64) * 64;
+    return c < ADJ_TJ ? ADJ_TJ : (c > ADJ_TJ_MAX ? ADJ_TJ_MAX : c);
+}
+
+__global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, int replica, int chunk,
+                                                          double *part_grad, double *param_part)
 {
     __shared__ float s_tab[REMC_NTYPES * REMC_NTYPES];
-    __shared__ float s_x[ADJ_TJ], s_y[ADJ_TJ], s_z[ADJ_TJ], s_q[ADJ_TJ], s_s[ADJ_TJ];
-    __shared__ int s_tc[ADJ_TJ], s_body[ADJ_TJ];
+    extern __shared__ float s_adj[]; // chunk x (x, y, z, q, sigma, type, body)
+    float *s_x = s_adj, *s_y = s_x + chunk, *s_z = s_y + chunk, *s_q = s_z + chunk, *s_s = s_q + chunk;
+    int *s_tc = (int *)(s_s + chunk), *s_body = s_tc + chunk;
     const int tid = threadIdx.x;
     const DevRep R = p.rep[replica];
     const DevSys S = p.sys[R.sys];
@@ -1599,9 +1609,9 @@ __global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, i
     const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int i = blockIdx.x * ADJ_TI + tid;
-    const int jc = blockIdx.y, j0 = jc * ADJ_TJ;
-    const int nj = min(ADJ_TJ, N - j0);
-    for (int k = tid; k < ADJ_TJ; k += ADJ_TI) {
+    const int jc = blockIdx.y, j0 = jc * chunk;
+    const int nj = min(chunk, N - j0);
+    for (int k = tid; k < chunk; k += ADJ_TI) {
         const int jt = j0 + k;
         if (k < nj) {
             s_x[k] = gx[jt]; s_y[k] = gy[jt]; s_z[k] = gz[jt];
@@ -1725,13 +1735,18 @@ size_t adjoint_scratch_doubles(int max_beads, int max_bodies)
 hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
                           int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s)
 {
-    const int ti = (n_beads + ADJ_TI - 1) / ADJ_TI, tj = (n_beads + ADJ_TJ - 1) / ADJ_TJ;
-    const size_t mti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, mtj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
+    // chunk >= ADJ_TJ, so the scratch sized for ADJ_TJ chunks of the largest system holds every split
+    const int chunk = adjoint_chunk(n_beads);
+    const int ti = (n_beads + ADJ_TI - 1) / ADJ_TI, tj = (n_beads + chunk - 1) / chunk;
+    const size_t mtj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
     double *part_grad = scratch, *bead_grad = part_grad + 3 * mtj * (size_t)max_beads;
     double *param_part = bead_grad + 3 * (size_t)max_beads;
-    (void)mti;
     const AdjFF af = *(const AdjFF *)adj_ff;
-    hipLaunchKernelGGL(k_adjoint_pairs, dim3(ti, tj), dim3(ADJ_TI), 0, s, p, af, replica, part_grad, param_part);
+    const size_t lds = 28 * (size_t)chunk;
+    hipError_t e = hipFuncSetAttribute((const void *)k_adjoint_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_adjoint_pairs, dim3(ti, tj), dim3(ADJ_TI), lds, s, p, af, replica, chunk, part_grad,
+                       param_part);
     hipLaunchKernelGGL(k_adjoint_reduce, dim3((3 * n_beads + 255) / 256), dim3(256), 0, s, n_beads, tj, part_grad,
                        bead_grad);
     hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 63) / 64), dim3(64), 0, s, p, replica, bead_grad, param_part,
