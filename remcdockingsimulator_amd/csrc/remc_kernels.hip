@@ -1,11 +1,15 @@
 // remc_kernels.hip -- gfx950 kernels of the REMC hot path.
 //
-//   k_mc<BLOCK>        batched persistent MC: one workgroup per replica runs
-//                      nsteps steps (proposal, dE of the moved body against all
-//                      other beads, FP64 Metropolis, commit) without leaving
-//                      the GPU.  Replaces Replica::MCSearch's per-step
+//   k_mc<BLOCK,CACHE,SPEC>  batched persistent MC: 1-8 workgroups per replica
+//                      run nsteps steps (proposal, dE of the moved body against
+//                      all other beads, FP64 Metropolis, commit) without
+//                      leaving the GPU, small moves in speculative windows.
+//                      Replaces Replica::MCSearch's per-step
 //                      MoleculeDataToDevice/EonDevice round trip
 //                      (Replica.cpp:368-478).
+//   k_mc_fb<BLOCK>     force-biased (smart) MC steps, one workgroup per replica.
+//   k_adjoint_*        gradient of one replica's energy: per-body force and
+//                      torque, d/d(lambda, e0, xi).
 //   k_energy_full      batched full energy, every replica in one launch,
 //                      256-bead tiles, half-window tile schedule (each
 //                      unordered tile pair once), Kahan FP32 per thread,
@@ -20,7 +24,7 @@
 // Pair term (Replica.cpp:584-600,694-733; CudaFunctions.cu:430-452,536-649):
 // minimum image, r = |d|; crowder pairs c*(6/r)^6 for r < 15; else
 // E_ij*(...) contact term with the repulsive flip plus Debye-Hueckel.  The
-// table etab already holds lambda*(T_ij - e0)*lj_cf*kbt_cf, the DH
+// table etab already holds 4*lambda*(T_ij - e0)*lj_cf*kbt_cf, the DH
 // coefficient dh_cf*kbt_cf, so a pair's value is directly kcal/mol.
 // x^6 is formed by multiplication; exp(-r/xi) = exp2(r*k_dh) with the
 // hardware exp2; 1/r from the hardware reciprocal square root.
@@ -86,7 +90,8 @@ __device__ __forceinline__ double wave_sum(double v)
 // the members of one replica share an XCD and its L2).  Member c keeps a
 // private copy c of the replica's dynamic state and evaluates dE over the
 // partner-bead slice [c*N/coop, (c+1)*N/coop).  Members publish their partial
-// dE per step in a 64-bit slot (value with its low 20 bits replaced by the step
+// dE per split step (per window: one per slot) in a 64-bit slot (value with
+// its low 20 bits replaced by the step
 // tag, written with one atomic store) and every member adds the coop partials
 // in member order, so all members take the identical decision and update
 // their copies identically.  With coop = 1 no slot is used.  A member that
@@ -214,9 +219,10 @@ size_t mc_dynamic_lds(const KParams &p)
 }
 
 // One launch runs nsteps MC steps of every replica.  A step whose moved body
-// is small (at most SPEC_MAX beads and not split across members) may be taken
-// in a speculative window when SPEC is set: the next w <= NW such steps are
-// evaluated at once, one wave per step, each against the conformation at the
+// is small (at most SPEC_MAX beads) may be taken in a speculative window when
+// SPEC is set: the next w <= NW such steps are evaluated at once, one wave
+// per step (with cooperating members, each over its partner slice, the
+// partials exchanged once per window), each against the conformation at the
 // window's start (dE0_k).  The exact dE of step k after the accepted steps
 // j < k of the window differs from dE0_k only by the pair terms between the
 // two moved bodies,
