@@ -241,8 +241,9 @@ REMC_API int remc_get_mc_block(remc_ctx *ctx, int *block);
 /* Workgroups per replica of the MC kernel (0 = autotune; else 1, 2, 4 or 8).
  * With more than one, each workgroup keeps a private copy of its replica's
  * state, evaluates dE over a slice of the partner beads, and the members add
- * their partials through device memory every step; the grid must fit
- * co-resident on the device (checked). */
+ * their partials through device memory (once per split step, once per
+ * speculative window); the grid must fit co-resident on the device
+ * (checked). */
 REMC_API int remc_set_mc_coop(remc_ctx *ctx, int coop);
 REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
 /* MC kernel tuning: lds_cache = 0 keeps partner beads in HBM instead of the
@@ -250,12 +251,14 @@ REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
  * split across cooperating workgroups (default 8192; 0 keeps the current). */
 REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
 /* Speculative windows of the MC kernel (default on): up to one step per wave
- * of consecutive moves of small bodies (at most 16 beads, not split across
- * workgroups) is evaluated at once against the window's starting
- * conformation and decided in order with exact pair corrections between the
- * window's moved bodies; used when the replica's beads and bodies fit the
- * LDS caches.  Decisions and states equal those of one step at a time up to
- * FP32 rounding.  0 turns the windows off.  No counterpart in the reference. */
+ * of consecutive moves of small bodies (at most 16 beads) is evaluated at
+ * once against the window's starting conformation (with cooperating
+ * workgroups, each over its partner slice); the window ends before the first
+ * step that an accepted earlier step of the window interacts with (exact pair
+ * corrections between the moved bodies) or whose body it moved.  Used when a
+ * partner cache fits in LDS (the replica, or each member's slice).  Decisions
+ * and states equal those of one step at a time up to FP32 rounding.  0 turns
+ * the windows off.  No counterpart in the reference. */
 REMC_API int remc_set_mc_window(remc_ctx *ctx, int enable);
 
 #ifdef __cplusplus
