@@ -353,3 +353,54 @@ def test_mc_uneven_systems_in_one_launch(oracle):
             assert abs(full[r] - want) <= 1e-5 * abs(want) + 1e-7
     e.close()
     assert diverged <= 1
+
+
+def _quat_rotate(q, v):
+    """Rotation of row vectors v by unit quaternions q (w, x, y, z), FP64."""
+    w, x, y, z = q[:, 0:1], q[:, 1:2], q[:, 2:3], q[:, 3:4]
+    u = np.concatenate([x, y, z], 1)
+    t = 2.0 * np.cross(u, v)
+    return v + w * t + np.cross(u, t)
+
+
+def test_mc_full_size_rigid_body_invariants(oracle):
+    """Config 5 at full size (11000 beads, 64 replicas) after 3000 steps of
+    windows, split chain moves and exchanges: size-independent properties
+    of rigid-body MC hold for every replica -- positions are relative
+    positions plus centres (bit for bit), centres lie in [0, L), rotations
+    stay unit quaternions, relative positions are the initial ones rotated by
+    the accumulated rotation, and the tracked energy follows the full one."""
+    wl = synth.config(5)
+    sa = wl.systems[0]
+    e = engine_for(wl, seed=SEED)
+    e.run(3, 1000)
+    e.synchronize()
+    body = np.repeat(np.arange(sa.n_bodies), np.diff(sa.body_start))
+    q0 = np.asarray(sa.quat, np.float64)
+    q0_inv = q0 * np.array([1.0, -1.0, -1.0, -1.0])
+    full = e.energy_all()
+    tracked = e.energies()
+    temps = e.temperatures()
+    moved = 0
+    for r in range(wl.n_replicas):
+        st = e.state(r)
+        assert np.array_equal(st["pos"], (st["rel"] + st["center"][body]).astype(np.float32)), r
+        assert np.all(st["center"] >= 0.0) and np.all(st["center"] < sa.box), r
+        q = st["quat"]
+        assert np.max(np.abs(np.sum(q * q, 1) - 1.0)) < 1e-12, r
+        # accumulated rotation q q0^-1 applied to the initial relative positions
+        w1, x1, y1, z1 = q.T
+        w2, x2, y2, z2 = q0_inv.T
+        dq = np.stack([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                       w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2], 1)
+        want = _quat_rotate(dq[body], sa.rel.astype(np.float64))
+        assert np.max(np.abs(st["rel"] - want)) < 2e-3, (r, np.max(np.abs(st["rel"] - want)))
+        moved += int(np.sum(np.any(st["center"] != sa.center, 1)))
+        assert abs(tracked[r] - full[r]) <= 1e-5 * abs(full[r]) + 1e-4 * kT(temps[r]) * 10, r
+    assert moved > wl.n_replicas * 100  # the runs moved many bodies
+    # the full energies themselves against the FP64 oracle on two replicas
+    o = oracle.OracleSystem(sa, wl.ff)
+    for r in (0, wl.n_replicas - 1):
+        want = o.energy(e.state(r)["pos"])
+        assert abs(full[r] - want) <= 1e-5 * abs(want)
+    e.close()
