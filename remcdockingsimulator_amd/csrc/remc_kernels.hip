@@ -163,7 +163,8 @@ struct StepDraw {
     float t[3];      // translation vector of a translate move
     int body;
     int rotate;
-    int m0, nm; // the moved body's first bead and bead count
+    int m0, nm;       // the moved body's first bead and bead count
+    int prev1, prev2; // the latest two earlier steps of the batch moving the same body (-1: none)
 };
 constexpr int DRAW_BATCH = 64;
 
@@ -250,6 +251,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ double s_wq[SPEC ? 4 * NW : 1];              // new rotation of a rotated slot
     __shared__ double s_wbase[SPEC ? NW : 1], s_wdE[SPEC ? NW : 1];
     __shared__ unsigned int s_wint[SPEC ? NW : 1]; // bit j of slot k: slot j's move changes slot k's dE
+    // bit j of slot k: slot k moves the body slot j moved (j the only earlier
+    // such slot), and is staged from slot j's proposal
+    __shared__ unsigned int s_wchain[SPEC ? NW : 1];
+    __shared__ int s_wnext[SPEC ? NW : 1]; // the accepted slot chained to slot j, or -1
     __shared__ int s_wdec[SPEC ? NW : 1];
     __shared__ int s_weff;
     __shared__ double s_E;
@@ -342,17 +347,25 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     int b0 = 0, have = 0; // s_draw holds the draws of steps [b0, b0 + have) of this launch
     int it = 0;
     while (it < nsteps) {
+        // The thread index is re-read opaquely in every step, so that the
+        // per-thread LDS addresses derived from it are formed where they are
+        // used (one or two instructions each) instead of being hoisted out of
+        // the loop, where they outnumber the registers and go to scratch.
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63, wave = tid >> 6;
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
         int off = it - b0;
         if (off >= have || (SPEC && off + NW > have && b0 + have < nsteps)) {
             // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
             if (wave == 0) {
                 bool elig = false;
+                int body_l = -1;
                 if (it + lane < nsteps) {
                     const remc_draws dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
                     StepDraw &sd = s_draw[lane];
                     sd.nlu = remc_neglog_u(dd.u);
-                    sd.body = dd.body;
+                    sd.body = body_l = dd.body;
                     sd.rotate = dd.rotate;
                     if (dd.rotate) {
                         double qq[4], cf[9];
@@ -371,6 +384,28 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (SPEC) {
                     const unsigned long long m = __ballot(elig);
                     if (lane == 0) s_elig = m;
+                    // the latest two earlier steps of the batch that move the same
+                    // body: the lanes whose body index agrees with this lane's in
+                    // every bit (one ballot per bit); lanes past the launch's last
+                    // step lie above every valid lane, so never count as earlier
+                    const int nbits = S.n_bodies > 1 ? 32 - __builtin_clz((unsigned int)(S.n_bodies - 1)) : 0;
+                    unsigned long long same = ~0ull;
+                    for (int bt = 0; bt < nbits; ++bt) {
+                        const bool on = ((body_l >> bt) & 1) != 0;
+                        const unsigned long long mb = __ballot(on);
+                        same &= on ? mb : ~mb;
+                    }
+                    const unsigned long long below = same & ((1ull << lane) - 1ull);
+                    int p1 = -1, p2 = -1;
+                    if (below != 0ull) {
+                        p1 = 63 - __builtin_clzll(below);
+                        const unsigned long long rest = below & ~(1ull << p1);
+                        if (rest != 0ull) p2 = 63 - __builtin_clzll(rest);
+                    }
+                    if (it + lane < nsteps) {
+                        s_draw[lane].prev1 = p1;
+                        s_draw[lane].prev2 = p2;
+                    }
                 }
             }
             b0 = it;
@@ -395,22 +430,42 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     const StepDraw &Dk = s_draw[off + k];
                     const int bk = Dk.body;
                     const int m0k = Dk.m0, nmk = Dk.nm;
+                    // One earlier slot j of the window moves the same body: slot k
+                    // is chained, staged from the state j's proposal leaves, and
+                    // stands if j is accepted.  Two or more: staged from the
+                    // window's start, it stands only if all of them are rejected.
+                    const int pj = Dk.prev1 - off;
+                    const bool chained = pj >= 0 && Dk.prev2 < off;
+                    if (i == 0) {
+                        s_wchain[k] = chained ? 1u << pj : 0u;
+                        s_wnext[k] = -1;
+                    }
                     if (i < nmk) {
                         // the moved beads come from the partner cache when it holds
                         // them (a member's slice may not), else from HBM
                         const int b = m0k + i;
                         const bool inc = b >= clo && b < chi;
-                        const float rx = cache_rel ? s_rx[b] : grx[b];
-                        const float ry = cache_rel ? s_ry[b] : gry[b];
-                        const float rz = cache_rel ? s_rz[b] : grz[b];
-                        float nrx = rx, nry = ry, nrz = rz;
-                        float ncx = cache_bodies ? s_cx[bk] : gcx[bk], ncy = cache_bodies ? s_cy[bk] : gcy[bk],
-                              ncz = cache_bodies ? s_cz[bk] : gcz[bk];
+                        float rx = cache_rel ? s_rx[b] : grx[b];
+                        float ry = cache_rel ? s_ry[b] : gry[b];
+                        float rz = cache_rel ? s_rz[b] : grz[b];
+                        float cx = cache_bodies ? s_cx[bk] : gcx[bk], cy = cache_bodies ? s_cy[bk] : gcy[bk],
+                              cz = cache_bodies ? s_cz[bk] : gcz[bk];
+                        float ox = inc ? s_jx[b - clo] : gx[b], oy = inc ? s_jy[b - clo] : gy[b],
+                              oz = inc ? s_jz[b - clo] : gz[b];
+                        if (chained) {
+                            // slot j's proposal, formed as slot j's own staging forms it
+                            const StepDraw &Dj = s_draw[off + pj];
+                            if (Dj.rotate) remc_qrotate(Dj.coef, rx, ry, rz, &rx, &ry, &rz);
+                            else remc_wrap_centre(cx, cy, cz, Dj.t[0], Dj.t[1], Dj.t[2], S.box, &cx, &cy, &cz);
+                            ox = rx + cx;
+                            oy = ry + cy;
+                            oz = rz + cz;
+                        }
+                        float nrx = rx, nry = ry, nrz = rz, ncx = cx, ncy = cy, ncz = cz;
                         if (Dk.rotate) remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
-                        else remc_wrap_centre(ncx, ncy, ncz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
+                        else remc_wrap_centre(cx, cy, cz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
                         const float qb = inc ? s_jq[b - clo] : qs[b];
-                        s_wold[tid] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
-                                          : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
+                        s_wold[tid] = make_float4(ox, oy, oz, qb * ff.c_dh);
                         s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
                         s_wrel[tid] = make_float4(nrx, nry, nrz, qb);
                         s_wtc[tid] = inc ? s_jtc[b - clo] : tcs[b];
@@ -418,6 +473,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                             if (Dk.rotate) {
                                 double qo[4], qn[4];
                                 for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * bk + t] : gq[4 * bk + t];
+                                if (chained && s_draw[off + pj].rotate) {
+                                    remc_qmul(s_draw[off + pj].q, qo, qn);
+                                    for (int t = 0; t < 4; ++t) qo[t] = qn[t];
+                                }
                                 remc_qmul(Dk.q, qo, qn);
                                 for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
                             } else {
@@ -478,7 +537,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (jj < kk && kk < w) {
                     const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
                     float cv = 0.0f;
-                    bool touches = bj == bk; // the same body: slot kk's proposal depends on slot jj's move
+                    // the same body: slot kk's proposal depends on slot jj's
+                    // move, unless kk is chained to jj (the decision handles it)
+                    bool touches = bj == bk && s_wchain[kk] == 0u;
                     if (bj != bk) {
                         const int nmj = s_draw[off + jj].nm, nmk = s_draw[off + kk].nm;
                         const bool crow = ((s_wtc[jj * SPEC_MAX] | s_wtc[kk * SPEC_MAX]) >> 8) != 0;
@@ -567,14 +628,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // Every slot is decided at once on its dE0.  That is step k's
                 // exact dE unless an accepted earlier slot j of the window moved a
                 // body that interacts with slot k's (corr(j,k) != 0) or moved the
-                // same body.  The window ends before the first such slot; the
-                // decisions before it are exact, and it starts the next window,
-                // staged from the updated conformation.
+                // same body, or slot k is chained to a slot j (staged from j's
+                // proposal) that is rejected.  The window ends before the first
+                // such slot; the decisions before it are exact, and it starts the
+                // next window, staged from the updated conformation.
                 const double nll = lane < w ? s_draw[off + lane].nlu : 0.0;
                 const int dl = lane < w ? remc_metropolis_l(dEl, beta, nll) : 0;
                 const unsigned long long acc0 = __ballot(dl != 0);
                 const unsigned int il = lane < w ? s_wint[lane] : 0u;
-                const bool inter = (il & (unsigned int)acc0) != 0u;
+                const unsigned int chl = lane < w ? s_wchain[lane] : 0u;
+                const bool inter = (il & (unsigned int)acc0) != 0u || (chl & ~(unsigned int)acc0) != 0u;
                 const unsigned long long stop = __ballot(inter) | (w < 64 ? (1ull << w) : 0ull);
                 const int weff = (int)__builtin_ctzll(stop);
                 if (lane < weff) {
@@ -597,6 +660,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     s_cnt[1] += (unsigned long long)nbz;
                     s_cnt[2] += (unsigned long long)(weff - nacc);
                 }
+                // an accepted chained slot supersedes its (accepted) predecessor's
+                // proposal in what the two moves both change
+                if (lane < weff && dl != 0 && chl != 0u) s_wnext[__builtin_ctz(chl)] = lane;
                 if (lane == 0) s_weff = weff;
             }
             __syncthreads();
@@ -604,26 +670,35 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (coop > 1) ++n_x;
             const int weff = s_weff;
 
-            // commit the accepted moves (their bodies are distinct): copies of
-            // what staging formed
+            // commit the accepted moves: copies of what staging formed.  Bodies
+            // are distinct except for a chained pair, where the later slot writes
+            // the positions and what its own move type changes (rotation: the
+            // relative positions and the quaternion; translation: the centre),
+            // and the earlier slot only what the later one leaves, so every
+            // location has one writer.
             if (tid < NW * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < weff && s_wdec[k]) {
                     const StepDraw &Dk = s_draw[off + k];
                     const int bk = Dk.body;
                     const int m0k = Dk.m0, nmk = Dk.nm;
+                    const int nx = s_wnext[k];
+                    const bool later = nx >= 0;
+                    const bool later_rot = later && s_draw[off + nx].rotate;
                     if (i < nmk) {
                         const int b = m0k + i;
-                        const float4 n = s_wnew[tid];
-                        gx[b] = n.x;
-                        gy[b] = n.y;
-                        gz[b] = n.z;
-                        if (b >= clo && b < chi) {
-                            s_jx[b - clo] = n.x;
-                            s_jy[b - clo] = n.y;
-                            s_jz[b - clo] = n.z;
+                        if (!later) {
+                            const float4 n = s_wnew[tid];
+                            gx[b] = n.x;
+                            gy[b] = n.y;
+                            gz[b] = n.z;
+                            if (b >= clo && b < chi) {
+                                s_jx[b - clo] = n.x;
+                                s_jy[b - clo] = n.y;
+                                s_jz[b - clo] = n.z;
+                            }
                         }
-                        if (Dk.rotate) {
+                        if (Dk.rotate && !later_rot) {
                             const float4 nr = s_wrel[tid];
                             grx[b] = nr.x;
                             gry[b] = nr.y;
@@ -636,12 +711,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         }
                         if (i == 0) {
                             if (Dk.rotate) {
-                                for (int t = 0; t < 4; ++t) {
-                                    const double qn = s_wq[4 * k + t];
-                                    gq[4 * bk + t] = qn;
-                                    if (cache_quat) s_q[4 * bk + t] = qn;
-                                }
-                            } else {
+                                if (!later_rot)
+                                    for (int t = 0; t < 4; ++t) {
+                                        const double qn = s_wq[4 * k + t];
+                                        gq[4 * bk + t] = qn;
+                                        if (cache_quat) s_q[4 * bk + t] = qn;
+                                    }
+                            } else if (!later || later_rot) {
                                 const float4 nc = s_wcen[k];
                                 gcx[bk] = nc.x;
                                 gcy[bk] = nc.y;

@@ -1,7 +1,7 @@
 """GPU energies through the C ABI against the FP64 oracle.
 
 Bar: 1e-5 relative on per-replica energies (absolute floor 1e-7 kcal/mol for
-near-zero totals), the north-star tolerance for FP32 pair terms with
+near-zero totals), the tolerance this path states for FP32 pair terms with
 compensated sums and FP64 totals."""
 import numpy as np
 import pytest

@@ -90,9 +90,72 @@ def test_mc_windows_off_matches_oracle(oracle, cid):
                       window=False) <= 1
 
 
+def repeats_in_window(bodies, width):
+    """Fraction of steps whose body was drawn exactly once in the previous
+    width - 1 steps (a slot chained to an earlier slot of its window), and
+    more than once (a slot that stands only if those are all rejected)."""
+    once = twice = 0
+    for k in range(len(bodies)):
+        n = int(np.sum(bodies[max(0, k - width + 1):k] == bodies[k]))
+        once += n == 1
+        twice += n > 1
+    return once / len(bodies), twice / len(bodies)
+
+
+@pytest.mark.parametrize("block,coop", [(256, 1), (1024, 1), (1024, 2)])
+def test_mc_windows_chained_repeats(oracle, block, coop):
+    """Six small bodies in a dilute box, nearly every move accepted: a window
+    often draws a body twice, and the later slot is staged from the earlier
+    slot's proposal (chained), so the window goes on past it when that
+    proposal is accepted; a third draw of the body falls back to ending the
+    window.  Decisions, dE and the bit-exact states must match the oracle's
+    one step at a time."""
+    from remcdockingsimulator_amd.engine import Engine
+    from remcdockingsimulator_amd.forcefield import ForceField
+
+    rng = np.random.default_rng(5)
+    ff = ForceField.synthetic(5)
+    bodies = [synth.chain_body(6, rng), synth.chain_body(9, rng)] + [synth.crowder_body() for _ in range(4)]
+    sa = synth.build_system(bodies, 70.0, rng)
+    lad = [300.0, 420.0, 600.0]
+    nsteps = 1200
+    e = Engine(0)
+    e.set_forcefield(ff)
+    e.set_seed(17)
+    sid = e.add_system(sa)
+    e.add_replicas(sid, lad, 1.5, 0.3)
+    e.set_mc_block(block)
+    e.set_mc_coop(coop)
+    e.set_recording(nsteps)
+    e.mc_steps(nsteps)
+    e.synchronize()
+    o = oracle.OracleSystem(sa, ff)
+    diverged = 0
+    for r in range(3):
+        say("chained windows block", block, "coop", coop, "replica", r)
+        orep = oracle.OracleReplica(o, lad[r], 1.5, 0.3, r)
+        orec = orep.mc(seed=17, nsteps=nsteps, mode=0)
+        once, twice = repeats_in_window(orec["body"], block // 64)
+        assert once > 0.15 and twice > 0.05, (once, twice)  # the draws exercise both kinds
+        assert np.mean(orec["accepted"] != 0) > 0.8
+        # the chains meet in steep contacts (dE of tens of kT), where FP32
+        # pair terms differ from FP64 by a few 1e-6 relative
+        n_eq, div = compare_decisions(e.records(r), orec, lad[r], rel_tol=1e-5)
+        if div:
+            diverged += 1
+            continue
+        st = e.state(r)
+        for key, want in (("pos", orep.pos), ("rel", orep.rel), ("center", orep.center), ("quat", orep.quat)):
+            assert np.array_equal(st[key], want), (r, key)
+        assert abs(e.energies()[r] - orep.energy) < 1e-4 * kT(lad[r])
+    assert diverged <= 1
+    e.close()
+
+
 def test_mc_windows_with_frequent_conflicts(oracle):
     """Three small bodies at high temperatures: most speculative windows end
-    early on a body an accepted step of the window already moved, and most
+    early, on a body drawn a third time, on a rejected move a later slot was
+    chained to, or on an accepted move of an interacting body; most
     corrections between slots are non-zero."""
     from remcdockingsimulator_amd.engine import Engine
     from remcdockingsimulator_amd.forcefield import ForceField
