@@ -250,15 +250,18 @@ REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
  * LDS cache; split_min_pairs (> 0) is the pair-term count from which a move is
  * split across cooperating workgroups (default 8192; 0 keeps the current). */
 REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
-/* Speculative windows of the MC kernel (default on): up to one step per wave
- * of consecutive moves of small bodies (at most 16 beads) is evaluated at
- * once against the window's starting conformation (with cooperating
- * workgroups, each over its partner slice); the window ends before the first
- * step that an accepted earlier step of the window interacts with (exact pair
- * corrections between the moved bodies) or whose body it moved.  Used when a
- * partner cache fits in LDS (the replica, or each member's slice).  Decisions
- * and states equal those of one step at a time up to FP32 rounding.  0 turns
- * the windows off.  No counterpart in the reference. */
+/* Speculative windows of the MC kernel (default on): up to two steps per wave
+ * (32 with 1024-thread workgroups) of consecutive moves of small bodies (at
+ * most 16 beads) are evaluated at once against the window's starting
+ * conformation (with cooperating workgroups, each over its partner slice); the
+ * window ends before the first step that an accepted earlier step of the
+ * window interacts with (exact pair corrections between the moved bodies) or
+ * whose body it moved.  A step whose body exactly one earlier step of the
+ * window moved is proposed from that step's proposal instead, and ends the
+ * window only if that step is rejected.  Used when a partner cache fits in
+ * LDS (the replica, or each member's slice).  Decisions and states equal those
+ * of one step at a time up to FP32 rounding.  0 turns the windows off.  No
+ * counterpart in the reference. */
 REMC_API int remc_set_mc_window(remc_ctx *ctx, int enable);
 
 #ifdef __cplusplus

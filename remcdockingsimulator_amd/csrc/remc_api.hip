@@ -504,12 +504,14 @@ int check_err(remc_ctx *c)
 // always; the partner beads of the replica (or of the member's slice when
 // every move is split) and the body table when they fit the LDS share of one
 // workgroup at the occupancy the grid needs.
-void fill_mc_lds(remc_ctx *c, KParams &p, int coop)
+void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
 {
     const int R = (int)c->reps.size();
     const long long wgs = (long long)R * coop;
     const int per_cu = (int)std::max<long long>(1, (wgs + c->n_cu - 1) / c->n_cu);
-    const int static_lds = 28 * 1024; // etab, draws, window staging, reduction scratch and slack
+    // etab, draws, window staging and reduction scratch, as the kernel declares them, and slack
+    const int kernel_lds = mc_static_lds(block);
+    const int static_lds = (kernel_lds > 0 ? kernel_lds : 48 * 1024) + 2048;
     long long budget = std::min<long long>(c->lds_per_block, c->lds_per_cu / per_cu) - static_lds;
     p.lds_mb = std::max(1, c->max_body_beads);
     budget -= 36LL * p.lds_mb;
@@ -560,7 +562,7 @@ int max_coop(remc_ctx *c, int block, int *out)
     int m = REMC_MAX_COOP;
     for (; m > 1; m >>= 1) {
         KParams p = c->params();
-        fill_mc_lds(c, p, m);
+        fill_mc_lds(c, p, m, block);
         int per_cu = 0;
         HIP_TRY(mc_max_active_per_cu(p, block, &per_cu));
         if ((long long)m * R <= (long long)per_cu * c->n_cu) break;
@@ -574,7 +576,7 @@ int mc_launch(remc_ctx *c, const KParams &p0, int nsteps, int block, int coop)
     int rc = sync_copies(c, coop);
     if (rc != REMC_OK) return rc;
     KParams p = p0;
-    fill_mc_lds(c, p, coop);
+    fill_mc_lds(c, p, coop, block);
     HIP_TRY(launch_mc(p, (int)c->reps.size(), nsteps, block, coop, c->stream));
     c->synced_copies = coop;
     return REMC_OK;
@@ -1097,7 +1099,7 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
     if (!c->graph || c->graph_steps != steps_per_round) {
         destroy_graph(c);
         KParams p = c->params();
-        fill_mc_lds(c, p, c->mc_coop);
+        fill_mc_lds(c, p, c->mc_coop, c->mc_block);
         int per_cu = 0; // also sets the kernel's dynamic-LDS attribute before capture
         HIP_TRY(mc_max_active_per_cu(p, c->mc_block, &per_cu));
         hipGraph_t g = nullptr;

@@ -101,6 +101,8 @@ struct KParams {
 // launchers (remc_kernels.hip)
 hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
 size_t mc_dynamic_lds(const KParams &p);
+// static LDS of the MC kernel at this block size (its largest variant), or -1
+int mc_static_lds(int block);
 hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s);
 size_t mc_fb_dynamic_lds(const KParams &p);
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
@@ -121,7 +123,7 @@ void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, do
 constexpr int FULL_TILE = 256;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
-constexpr int REMC_SLOT_K = 16;  // partial sums per member and exchange (window slots; 1 for a split step)
+constexpr int REMC_SLOT_K = 32;  // partial sums per member and exchange (window slots; 1 for a split step)
 // two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica
 constexpr int REMC_SLOTS_PER_REP = 2 * REMC_SLOT_K * REMC_MAX_COOP;
 

@@ -221,24 +221,28 @@ size_t mc_dynamic_lds(const KParams &p)
 
 // One launch runs nsteps MC steps of every replica.  A step whose moved body
 // is small (at most SPEC_MAX beads) may be taken in a speculative window when
-// SPEC is set: the next w <= NW such steps are evaluated at once, one wave
-// per step (with cooperating members, each over its partner slice, the
-// partials exchanged once per window), each against the conformation at the
-// window's start (dE0_k).  The exact dE of step k after the accepted steps
-// j < k of the window differs from dE0_k only by the pair terms between the
-// two moved bodies,
+// SPEC is set: the next w <= WK such steps are evaluated at once, two slots per
+// wave (with cooperating members, each over its partner slice, the partials
+// exchanged once per window), each against the conformation at the window's
+// start (dE0_k).  The exact dE of step k after the accepted steps j < k of the
+// window differs from dE0_k only by the pair terms between the two moved
+// bodies,
 //   corr(j,k) = sum_{i in k, l in j} [e(n_i,n_l) - e(o_i,n_l)] - [e(n_i,o_l) - e(o_i,o_l)],
 // which are formed for every pair of slots in parallel.  Every slot is decided
 // on its dE0; the window ends before the first slot k that an accepted earlier
 // slot j interacts with (corr(j,k) != 0, or the same body, whose proposal
 // depends on j's move), so every kept decision used its exact dE; slot k starts
-// the next window.  Decisions, energies and states are those of taking the
-// steps one at a time, up to the FP32 rounding of the reassociated pair sums.
+// the next window.  A slot whose body exactly one earlier slot j moved is
+// staged from j's proposal instead and ends the window only if j is rejected.
+// Decisions, energies and states are those of taking the steps one at a time,
+// up to the FP32 rounding of the reassociated pair sums.
 template <int BLOCK, bool CACHE, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
-    constexpr int WS = SPEC ? NW * SPEC_MAX : 1;
+    constexpr int WK = SPEC ? 2 * NW : 1; // window slots: at most 32 (the slot masks are 32-bit)
+    static_assert(WK <= 32 && WK <= REMC_SLOT_K, "window slots");
+    constexpr int WS = SPEC ? WK * SPEC_MAX : 1;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[NW];
     __shared__ double s_dE;
@@ -247,15 +251,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
     __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old (w: q c_dh) / new (w: sigma) position, new rel (w: q)
     __shared__ int s_wtc[WS];                             // type | crowder << 8 of the staged beads
-    __shared__ float4 s_wcen[SPEC ? NW : 1];                // new centre of a translated slot
-    __shared__ double s_wq[SPEC ? 4 * NW : 1];              // new rotation of a rotated slot
-    __shared__ double s_wbase[SPEC ? NW : 1], s_wdE[SPEC ? NW : 1];
-    __shared__ unsigned int s_wint[SPEC ? NW : 1]; // bit j of slot k: slot j's move changes slot k's dE
+    __shared__ float4 s_wcen[WK];                         // new centre of a translated slot
+    __shared__ double s_wq[4 * WK];                       // new rotation of a rotated slot
+    __shared__ double s_wbase[WK], s_wdE[WK];
+    __shared__ unsigned int s_wint[WK]; // bit j of slot k: slot j's move changes slot k's dE
     // bit j of slot k: slot k moves the body slot j moved (j the only earlier
     // such slot), and is staged from slot j's proposal
-    __shared__ unsigned int s_wchain[SPEC ? NW : 1];
-    __shared__ int s_wnext[SPEC ? NW : 1]; // the accepted slot chained to slot j, or -1
-    __shared__ int s_wdec[SPEC ? NW : 1];
+    __shared__ unsigned int s_wchain[WK];
+    __shared__ int s_wnext[WK]; // the accepted slot chained to slot j, or -1
+    __shared__ int s_wdec[WK];
     __shared__ int s_weff;
     __shared__ double s_E;
     __shared__ unsigned long long s_cnt[3];
@@ -356,7 +360,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         const int lane = tid & 63, wave = tid >> 6;
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
         int off = it - b0;
-        if (off >= have || (SPEC && off + NW > have && b0 + have < nsteps)) {
+        bool refill = off >= have;
+        if (SPEC && !refill && off + WK > have && b0 + have < nsteps) {
+            // fewer than WK draws left: form the next batch now if the run of
+            // window-eligible steps from here reaches the batch's end (else a
+            // large move ends the window before it anyway)
+            const unsigned long long rest = (1ull << (have - off)) - 1ull; // have - off < WK <= 32
+            refill = ((~(s_elig >> off)) & rest) == 0ull;
+        }
+        if (refill) {
             // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
             if (wave == 0) {
                 bool elig = false;
@@ -377,8 +389,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
                     }
                     const int gb = S.body_off + dd.body;
-                    sd.m0 = p.bstart[gb];
-                    sd.nm = p.bend[gb] - sd.m0;
+                    sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
+                    sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
                     if (SPEC) elig = sd.nm <= SPEC_MAX;
                 }
                 if (SPEC) {
@@ -418,13 +430,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         if (SPEC) {
             const unsigned long long run = ~(s_elig >> off);
             w = run == 0ull ? 64 : (int)__builtin_ctzll(run);
-            w = min(w, NW);
+            w = min(w, WK);
         }
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
             // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
-            if (tid < NW) s_wint[tid] = 0u;
-            if (tid < NW * SPEC_MAX) {
+            if (tid < WK) s_wint[tid] = 0u;
+            if (tid < WK * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < w) {
                     const StepDraw &Dk = s_draw[off + k];
@@ -488,14 +500,53 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             }
             __syncthreads();
 
-            // dE0 of slot k = wave k: its moved beads against every other bead;
-            // with cooperating members, against this member's partner slice,
-            // the members' partials then exchanged once for the whole window
+            // dE0 of slots k = wave and wave + NW: the moved beads against every
+            // other bead; with cooperating members, against this member's
+            // partner slice, the members' partials then exchanged once for the
+            // whole window
             const unsigned long long wtag = step & 0x7FFFFull;
             unsigned long long *bank = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
             const int wjs = coop > 1 ? cjs : 0, wje = coop > 1 ? cje : N;
-            if (wave < w) {
-                const int k = wave;
+            int k0 = wave;
+            if (wave + NW < w && s_draw[off + wave].nm == 1 && s_draw[off + wave + NW].nm == 1 &&
+                ((s_wtc[wave * SPEC_MAX] & s_wtc[(wave + NW) * SPEC_MAX]) >> 8) != 0) {
+                // both slots of the wave move one crowder bead: one pass over the
+                // partners serves both (the same per-lane order of terms as below)
+                const int ka = wave, kb = wave + NW;
+                const float4 oa = s_wold[ka * SPEC_MAX], na = s_wnew[ka * SPEC_MAX];
+                const float4 ob = s_wold[kb * SPEC_MAX], nb = s_wnew[kb * SPEC_MAX];
+                const int ba = s_draw[off + ka].m0, bb = s_draw[off + kb].m0;
+                float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
+                for (int j = wjs + lane; j < wje; j += 64) {
+                    const int jc = j - clo;
+                    const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                    if (j != ba) {
+                        const float eo = pair_crowder(dist2(xj, yj, zj, oa, S), ff);
+                        const float en = pair_crowder(dist2(xj, yj, zj, na, S), ff);
+                        kahan_add(acc_a, comp_a, en - eo);
+                    }
+                    if (j != bb) {
+                        const float eo = pair_crowder(dist2(xj, yj, zj, ob, S), ff);
+                        const float en = pair_crowder(dist2(xj, yj, zj, nb, S), ff);
+                        kahan_add(acc_b, comp_b, en - eo);
+                    }
+                }
+                const double va = wave_sum_dpp((double)acc_a - (double)comp_a);
+                const double vb = wave_sum_dpp((double)acc_b - (double)comp_b);
+                if (lane == 0) {
+                    if (coop > 1) {
+                        __hip_atomic_store(bank + ka * REMC_MAX_COOP + c, slot_pack(va, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(bank + kb * REMC_MAX_COOP + c, slot_pack(vb, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    } else {
+                        s_wbase[ka] = va;
+                        s_wbase[kb] = vb;
+                    }
+                }
+                k0 = w; // both done
+            }
+            for (int k = k0; k < w; k += NW) {
                 const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
                 const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
                 float acc = 0.0f, comp = 0.0f;
@@ -532,8 +583,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
             }
             // pair corrections between the moved bodies of every two slots j < k
-            if (tid < NW * NW) {
-                const int jj = tid / NW, kk = tid % NW;
+            if (tid < WK * WK) {
+                const int jj = tid / WK, kk = tid % WK;
                 if (jj < kk && kk < w) {
                     const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
                     float cv = 0.0f;
@@ -581,24 +632,29 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (wave == 0) {
                 double dEl = 0.0;
                 if (coop > 1) {
-                    // gather the w x coop partials (item t = k * coop + m, two per lane)
-                    const int nt = w * coop;
-                    const int t0 = lane, t1 = lane + 64;
-                    unsigned long long v0 = 0ull, v1 = 0ull;
-                    bool ok0 = t0 >= nt, ok1 = t1 >= nt;
+                    // lane k gathers slot k's coop partials (bank[k][m]) and adds
+                    // them in member order: the same total in every member
+                    unsigned long long v[REMC_MAX_COOP];
+                    bool ok[REMC_MAX_COOP];
+#pragma unroll
+                    for (int m = 0; m < REMC_MAX_COOP; ++m) {
+                        v[m] = 0ull;
+                        ok[m] = lane >= w || m >= coop;
+                    }
                     int spins = 0;
                     bool failed = false;
-                    while (!__all(ok0 && ok1)) {
-                        if (!ok0) {
-                            v0 = __hip_atomic_load(bank + (t0 / coop) * REMC_MAX_COOP + t0 % coop, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                            ok0 = (v0 & SLOT_TAG_MASK) == wtag;
-                        }
-                        if (!ok1) {
-                            v1 = __hip_atomic_load(bank + (t1 / coop) * REMC_MAX_COOP + t1 % coop, __ATOMIC_RELAXED,
-                                                   __HIP_MEMORY_SCOPE_AGENT);
-                            ok1 = (v1 & SLOT_TAG_MASK) == wtag;
-                        }
+                    for (;;) {
+                        bool mine = true;
+#pragma unroll
+                        for (int m = 0; m < REMC_MAX_COOP; ++m) mine = mine && ok[m];
+                        if (__all(mine)) break;
+#pragma unroll
+                        for (int m = 0; m < REMC_MAX_COOP; ++m)
+                            if (!ok[m]) {
+                                v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+                                ok[m] = (v[m] & SLOT_TAG_MASK) == wtag;
+                            }
                         if (++spins >= SLOT_SPIN_LIMIT) {
                             failed = true;
                             break;
@@ -615,12 +671,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                             s_abort = 1;
                         }
                     }
-                    // slot k's total, members added in member order: identical in every member
-                    for (int m = 0; m < coop; ++m) {
-                        const int t = lane * coop + m;
-                        const unsigned long long a0 = __shfl(v0, t & 63, 64), a1 = __shfl(v1, t & 63, 64);
-                        dEl += slot_value(t < 64 ? a0 : a1);
-                    }
+#pragma unroll
+                    for (int m = 0; m < REMC_MAX_COOP; ++m)
+                        if (m < coop) dEl += slot_value(v[m]);
                     if (lane >= w) dEl = 0.0;
                 } else {
                     dEl = lane < w ? s_wbase[lane] : 0.0;
@@ -676,7 +729,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             // relative positions and the quaternion; translation: the centre),
             // and the earlier slot only what the later one leaves, so every
             // location has one writer.
-            if (tid < NW * SPEC_MAX) {
+            if (tid < WK * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < weff && s_wdec[k]) {
                     const StepDraw &Dk = s_draw[off + k];
@@ -1423,6 +1476,25 @@ hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out)
     case 512: return occupancy_b<512>(p, out);
     case 1024: return occupancy_b<1024>(p, out);
     default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BLOCK>
+static int static_lds_b()
+{
+    // the window variant holds the most: its slot staging on top of the rest
+    hipFuncAttributes a;
+    if (hipFuncGetAttributes(&a, (const void *)k_mc<BLOCK, true, true>) != hipSuccess) return -1;
+    return (int)a.sharedSizeBytes;
+}
+
+int mc_static_lds(int block)
+{
+    switch (block) {
+    case 256: return static_lds_b<256>();
+    case 512: return static_lds_b<512>();
+    case 1024: return static_lds_b<1024>();
+    default: return -1;
     }
 }
 
