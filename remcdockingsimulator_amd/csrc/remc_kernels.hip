@@ -349,6 +349,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __syncthreads();
 
     int b0 = 0, have = 0; // s_draw holds the draws of steps [b0, b0 + have) of this launch
+    unsigned long long elig = 0ull; // s_elig of the batch, read once per batch
     int it = 0;
     while (it < nsteps) {
         // The thread index is re-read opaquely in every step, so that the
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             // window-eligible steps from here reaches the batch's end (else a
             // large move ends the window before it anyway)
             const unsigned long long rest = (1ull << (have - off)) - 1ull; // have - off < WK <= 32
-            refill = ((~(s_elig >> off)) & rest) == 0ull;
+            refill = ((~(elig >> off)) & rest) == 0ull;
         }
         if (refill) {
             // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
@@ -424,11 +425,12 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             have = min(DRAW_BATCH, nsteps - it);
             off = 0;
             __syncthreads();
+            if (SPEC) elig = s_elig;
         }
 
         int w = 0;
         if (SPEC) {
-            const unsigned long long run = ~(s_elig >> off);
+            const unsigned long long run = ~(elig >> off);
             w = run == 0ull ? 64 : (int)__builtin_ctzll(run);
             w = min(w, WK);
         }
@@ -630,6 +632,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 
             // decisions in step order (wave 0); lane l carries slot l's dE
             if (wave == 0) {
+                // the slot's uniform and masks, read before the wait for the partials
+                const double nll = lane < w ? s_draw[off + lane].nlu : 0.0;
+                const unsigned int il = lane < w ? s_wint[lane] : 0u;
+                const unsigned int chl = lane < w ? s_wchain[lane] : 0u;
                 double dEl = 0.0;
                 if (coop > 1) {
                     // lane k gathers slot k's coop partials (bank[k][m]) and adds
@@ -685,11 +691,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // proposal) that is rejected.  The window ends before the first
                 // such slot; the decisions before it are exact, and it starts the
                 // next window, staged from the updated conformation.
-                const double nll = lane < w ? s_draw[off + lane].nlu : 0.0;
                 const int dl = lane < w ? remc_metropolis_l(dEl, beta, nll) : 0;
                 const unsigned long long acc0 = __ballot(dl != 0);
-                const unsigned int il = lane < w ? s_wint[lane] : 0u;
-                const unsigned int chl = lane < w ? s_wchain[lane] : 0u;
                 const bool inter = (il & (unsigned int)acc0) != 0u || (chl & ~(unsigned int)acc0) != 0u;
                 const unsigned long long stop = __ballot(inter) | (w < 64 ? (1ull << w) : 0ull);
                 const int weff = (int)__builtin_ctzll(stop);
