@@ -268,7 +268,9 @@ def main():
         fpairs = rngm.delta_pairs(SEED, range(R), fb_first, args.fb_steps, sizes[0], tag=rngm.TAG_FB)
         fb = {"mc_steps_per_s": R * args.fb_steps / fw,
               "dE_pair_interactions_per_s": fpairs / fw,
-              "kernel": "k_mc_fb (force-biased MC, one 512-thread workgroup per replica)",
+              "pair_evaluations_per_s": 2.0 * fpairs / fw,
+              "kernel": "k_mc_fb (force-biased MC, 512-thread workgroups, as many per replica as the MC kernel; "
+                        "large moves split over them)",
               "kernel_avg_ms": fms / max(fnl, 1), "launches": fnl, "steps": args.fb_steps,
               "work_per_step": "force at the old position, energy and force at the new, energy at the old: "
                                "2 n_m (N - n_m) pair energies and 2 n_m (N - n_m) pair derivatives"}
@@ -325,7 +327,10 @@ def main():
         "roofline": roofline,
         "plain": plain,
         "speedup_vs_plain": {"pair_interactions": value / plain["pair_interactions_per_s"],
-                             "mc_steps": (steps_total / wall) / plain["mc_steps_per_s"]} if plain else None,
+                             "mc_steps": (steps_total / wall) / plain["mc_steps_per_s"],
+                             "force_biased_pair_evaluations": (fb["pair_evaluations_per_s"]
+                                                               / plain["pair_interactions_per_s"]) if fb else None}
+        if plain else None,
         "cpu_baseline": cpu,
         "force_biased": fb,
         "adjoint": adj,

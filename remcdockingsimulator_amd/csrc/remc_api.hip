@@ -941,24 +941,43 @@ int remc_mc_steps_fb(remc_ctx *c, int nsteps, int block)
     if (block != 256 && block != 512 && block != 1024)
         return fail(REMC_ERR_ARG, "workgroup size %d (256, 512 or 1024)", block);
     if (nsteps == 0) return REMC_OK;
-    // copy 0 of every replica is current after any MC launch; the others go stale here
-    int rc = sync_copies(c, 1);
-    if (rc != REMC_OK) return rc;
     KParams p = c->params();
     p.lds_mb = std::max(1, c->max_body_beads);
     p.lds_cl = 0;
-    const long long static_lds = 12 * 1024; // table, reductions, 64 batched draws
+    const long long static_lds = 13 * 1024; // table, reductions, exchange, 64 batched draws
     if ((long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block)
         return fail(REMC_ERR_LIMIT, "a body of %d beads does not fit the force-biased kernel's LDS staging",
                     c->max_body_beads);
-    // the replica's partner beads in LDS when they fit (else read from HBM)
+    // the replica's partner beads with their relative positions, and its body
+    // centres and rotations, in LDS when they all fit (else read from HBM)
     p.lds_cl = c->max_beads;
-    if (!c->lds_cache || (long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block) p.lds_cl = 0;
+    p.lds_nbody = c->max_bodies;
+    if (!c->lds_cache || (long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block) {
+        p.lds_cl = 0;
+        p.lds_nbody = 0;
+    }
+    // windows of one-bead crowder moves (with the cache), switched with the MC kernel's
+    p.spec = c->mc_window ? 1 : 0;
+    // workgroups per replica: the MC kernel's setting when one is made, else
+    // as many as give every workgroup a CU of its own; never more than run
+    // co-resident (the members of a replica wait for each other)
+    const int R = (int)c->reps.size();
+    int coop = c->mc_coop > 0 ? c->mc_coop : REMC_MAX_COOP;
+    if (c->mc_coop <= 0)
+        while (coop > 1 && (long long)R * coop > (long long)c->n_cu) coop >>= 1;
+    if (coop > 1) {
+        int per_cu = 0;
+        HIP_TRY(mc_fb_max_active_per_cu(p, block, &per_cu));
+        while (coop > 1 && (long long)R * coop > (long long)per_cu * c->n_cu) coop >>= 1;
+    }
+    // every member starts from the same state
+    int rc = sync_copies(c, coop);
+    if (rc != REMC_OK) return rc;
     {
         TimedLaunch t(c, 0);
-        HIP_TRY(launch_mc_fb(p, (int)c->reps.size(), nsteps, block, c->stream));
+        HIP_TRY(launch_mc_fb(p, R, nsteps, block, coop, c->stream));
     }
-    c->synced_copies = 1;
+    c->synced_copies = coop;
     return REMC_OK;
 }
 

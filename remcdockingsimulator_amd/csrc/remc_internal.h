@@ -103,7 +103,8 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
 size_t mc_dynamic_lds(const KParams &p);
 // static LDS of the MC kernel at this block size (its largest variant), or -1
 int mc_static_lds(int block);
-hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s);
+hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
+hipError_t mc_fb_max_active_per_cu(const KParams &p, int block, int *out);
 size_t mc_fb_dynamic_lds(const KParams &p);
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,

@@ -1112,49 +1112,73 @@ __device__ __forceinline__ void pair_eg(float r2, bool crow, float eij, float si
     g = fmaxf(fminf((dlj + ddh) * rinv, 1e30f), -1e30f);
 }
 
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
+template <int BLOCK, bool CACHE>
+__global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[7][NW];
     __shared__ double s_ft[7];
-    __shared__ double s_disp[3], s_q[4], s_coef[9];
-    __shared__ float s_cen[3];
-    __shared__ int s_dec;
-    __shared__ double s_dE;
+    __shared__ int s_abort;
+    __shared__ double s_qr[4]; // the rotation of the step's proposal
+    __shared__ float s_ncen[3]; // the centre of the step's proposal
     __shared__ remc_fb_draws s_fbd[64];
+    __shared__ int s_fm0[64], s_fnm[64], s_fcrow[64]; // the drawn body's first bead, bead count, crowder flag
+    // windows of one-bead crowder moves (one slot per wave, at most FBW)
+    constexpr int FBW = NW < 8 ? NW : 8;
+    __shared__ unsigned long long s_felig;               // steps of the batch a window may take
+    __shared__ double s_wA[3 * FBW], s_wB[4 * FBW];      // force at the old position; dE and force at the new
+    __shared__ float4 s_wold[FBW], s_wnew[FBW], s_wnrel[FBW], s_wncen[FBW];
+    __shared__ double s_wdisp[3 * FBW], s_wfwd[FBW], s_wqp[4 * FBW], s_wdE[FBW];
+    __shared__ unsigned int s_wint[FBW];
+    __shared__ int s_wdec[FBW], s_weff;
     extern __shared__ float4 smem[];
     float4 *s_old = smem;                // old position, q_i c_dh
     float4 *s_orel = s_old + p.lds_mb;   // old relative position, sigma
     float4 *s_new = s_orel + p.lds_mb;   // new position
     float4 *s_nrel = s_new + p.lds_mb;   // new relative position
     int *s_tc = (int *)(s_nrel + p.lds_mb);
-    // partner cache (p.lds_cl beads: the whole replica, or 0 = read from HBM)
-    const bool cached = p.lds_cl > 0;
-    float *s_jx = (float *)(s_tc + p.lds_mb), *s_jy = s_jx + p.lds_cl, *s_jz = s_jy + p.lds_cl;
-    float *s_jq = s_jz + p.lds_cl, *s_js = s_jq + p.lds_cl;
-    int *s_jtc = (int *)(s_js + p.lds_cl);
+    // CACHE: the replica's partner beads with their relative positions
+    // (p.lds_cl of them) and its body centres and rotations (p.lds_nbody) in
+    // LDS; else all of them are read from HBM
+    constexpr bool cached = CACHE, cache_rel = CACHE, cache_bodies = CACHE;
+    const int cl = CACHE ? p.lds_cl : 0, nb_c = CACHE ? p.lds_nbody : 0;
+    float *s_jx = (float *)(s_tc + p.lds_mb), *s_jy = s_jx + cl, *s_jz = s_jy + cl;
+    float *s_jq = s_jz + cl, *s_js = s_jq + cl;
+    int *s_jtc = (int *)(s_js + cl);
+    float *s_rx = (float *)(s_jtc + cl), *s_ry = s_rx + cl, *s_rz = s_ry + cl;
+    char *const bq_at = (char *)(s_rz + cl);
+    double *s_bq = (double *)(bq_at + ((8 - ((bq_at - (char *)smem) & 7)) & 7)); // 8-byte aligned
+    float *s_bcx = (float *)(s_bq + 4 * nb_c), *s_bcy = s_bcx + nb_c, *s_bcz = s_bcy + nb_c;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int r = blockIdx.x;
+    const int r = blockIdx.x % p.n_rep;
+    const int c = blockIdx.x / p.n_rep; // member c of the replica's coop workgroups
     const DevRep R = p.rep[r];
     const DevSys S = p.sys[R.sys];
     const DevFF ff = p.ff;
     for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
+    if (tid == 0) s_abort = 0;
     const double beta = remc_beta(p.T[r]);
     const double a_t = (double)R.tstep * (double)R.tstep / 6.0, a_r = R.rstep * R.rstep / 6.0;
     const float inv_xi = -ff.k_dh * 0.69314718f; // 1/xi from k_dh = -log2(e)/xi
     unsigned long long step = p.step[r];
     double E = p.E[r];
     unsigned long long n_acc = 0, n_accb = 0, n_rej = 0;
-    float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
-    float *grx = p.rx + R.bead_off, *gry = p.ry + R.bead_off, *grz = p.rz + R.bead_off;
-    float *gcx = p.cx + R.body_off, *gcy = p.cy + R.body_off, *gcz = p.cz + R.body_off;
-    double *gq = p.quat + 4 * (size_t)R.body_off;
+    // member c's private copy of the replica's dynamic state; every member
+    // commits every move to its own copy
+    const size_t cb = (size_t)c * p.copy_beads + R.bead_off;
+    const size_t cmo = (size_t)c * p.copy_bodies + R.body_off;
+    float *gx = p.px + cb, *gy = p.py + cb, *gz = p.pz + cb;
+    float *grx = p.rx + cb, *gry = p.ry + cb, *grz = p.rz + cb;
+    float *gcx = p.cx + cmo, *gcy = p.cy + cmo, *gcz = p.cz + cmo;
+    double *gq = p.quat + 4 * cmo;
     const int *tcs = p.tc + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
     const int N = S.n_beads;
+    const int cjs = (int)(((long long)c * N) / coop), cje = (int)(((long long)(c + 1) * N) / coop);
+    unsigned long long *slots = p.slots + (size_t)r * REMC_SLOTS_PER_REP;
+    unsigned int n_x = 0; // exchanges so far in this launch: selects the slot bank
     if (cached)
         for (int b = tid; b < N; b += BLOCK) {
             s_jx[b] = gx[b];
@@ -1163,7 +1187,20 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
             s_jq[b] = qs[b];
             s_js[b] = sgs[b];
             s_jtc[b] = tcs[b];
+            if (cache_rel) {
+                s_rx[b] = grx[b];
+                s_ry[b] = gry[b];
+                s_rz[b] = grz[b];
+            }
         }
+    if (cache_bodies) {
+        for (int m = tid; m < S.n_bodies; m += BLOCK) {
+            s_bcx[m] = gcx[m];
+            s_bcy[m] = gcy[m];
+            s_bcz[m] = gcz[m];
+        }
+        for (int m = tid; m < 4 * S.n_bodies; m += BLOCK) s_bq[m] = gq[m];
+    }
     __syncthreads();
 
     // block sum of v[0..n) into s_ft (FP64, fixed order)
@@ -1180,37 +1217,376 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         }
         __syncthreads();
     };
+    // vals[0..n) (this member's partials, n <= REMC_SLOT_K) become the
+    // members' totals, added in member order so that every member holds the
+    // same values: one tagged 64-bit slot per value and member, the slot bank
+    // alternating with the exchange count (a member reuses a bank only after
+    // every member has read it), bounded waits; lane k of wave 0 gathers value k
+    auto exchange = [&](double *vals, int n, unsigned long long tag) {
+        unsigned long long *bank = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
+        if (wave == 0) {
+            if (lane < n)
+                __hip_atomic_store(bank + lane * REMC_MAX_COOP + c, slot_pack(vals[lane], tag), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            unsigned long long v[REMC_MAX_COOP];
+            bool ok[REMC_MAX_COOP];
+#pragma unroll
+            for (int m = 0; m < REMC_MAX_COOP; ++m) {
+                v[m] = 0ull;
+                ok[m] = lane >= n || m >= coop;
+            }
+            int spins = 0;
+            bool failed = false;
+            for (;;) {
+                bool mine = true;
+#pragma unroll
+                for (int m = 0; m < REMC_MAX_COOP; ++m) mine = mine && ok[m];
+                if (__all(mine)) break;
+#pragma unroll
+                for (int m = 0; m < REMC_MAX_COOP; ++m)
+                    if (!ok[m]) {
+                        v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+                        ok[m] = (v[m] & SLOT_TAG_MASK) == tag;
+                    }
+                if (++spins >= SLOT_SPIN_LIMIT) {
+                    failed = true;
+                    break;
+                }
+                if ((spins & 255) == 0 && __hip_atomic_load(p.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                    failed = true;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            if (failed) {
+                if (lane == 0) {
+                    __hip_atomic_fetch_or(p.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_abort = 1;
+                }
+            }
+            if (lane < n) {
+                double t = 0.0;
+#pragma unroll
+                for (int m = 0; m < REMC_MAX_COOP; ++m)
+                    if (m < coop) t += slot_value(v[m]);
+                vals[lane] = t;
+            }
+        }
+        __syncthreads();
+        ++n_x;
+    };
 
     for (int it = 0; it < nsteps; ++it, ++step) {
+        // the thread index re-read opaquely in every step, as in k_mc: the
+        // addresses derived from it are formed where used, not held in registers
+        int tid = threadIdx.x;
+        asm volatile("" : "+v"(tid));
+        const int lane = tid & 63, wave = tid >> 6;
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
         if ((it & 63) == 0) {
-            // wave 0 forms the draws of the next 64 steps, one per lane
-            if (wave == 0 && it + lane < nsteps) s_fbd[lane] = remc_fb_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+            // wave 0 forms the draws of the next 64 steps, one per lane, with the drawn body's bead range
+            if (wave == 0 && it + lane < nsteps) {
+                const remc_fb_draws dd = remc_fb_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+                s_fbd[lane] = dd;
+                const int gb = S.body_off + dd.body;
+                s_fm0[lane] = p.bstart[gb];
+                s_fnm[lane] = p.bend[gb] - p.bstart[gb];
+                s_fcrow[lane] = p.bcrow[gb];
+            }
+            if (wave == 0) {
+                const bool el = it + lane < nsteps && s_fnm[lane] == 1 && s_fcrow[lane] != 0;
+                const unsigned long long m = __ballot(el);
+                if (lane == 0) s_felig = m;
+            }
             __syncthreads();
+        }
+        // a window of w >= 2 one-bead crowder moves, within the batch
+        int w = 0;
+        if (CACHE && p.spec) {
+            const unsigned long long run = ~(s_felig >> (it & 63));
+            w = min(run == 0ull ? 64 : (int)__builtin_ctzll(run), FBW);
+        }
+        if (CACHE && w >= 2) {
+            // ------------- window of w one-bead crowder moves, slot k = wave k -------------
+            // Each slot's force, dE and new force are taken against the
+            // conformation at the window's start; a crowder's pair terms vanish
+            // beyond the cutoff, so they are exact unless an accepted earlier
+            // slot of the window moved within the cutoff of the slot's bead (old
+            // or new position, either end) or moved the same body: the window
+            // ends before the first such slot, which starts the next one.
+            const int js = coop > 1 ? cjs : 0, je = coop > 1 ? cje : N;
+            const int bw = it & 63;
+            if (wave < w) {
+                const int k = wave, b = s_fm0[bw + k];
+                const float ox = s_jx[b], oy = s_jy[b], oz = s_jz[b];
+                double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+                for (int j = js + lane; j < je; j += 64) {
+                    if (j == b) continue;
+                    const float dx = min_image(ox - s_jx[j], S.box, S.inv_box);
+                    const float dy = min_image(oy - s_jy[j], S.box, S.inv_box);
+                    const float dz = min_image(oz - s_jz[j], S.box, S.inv_box);
+                    float e, g2;
+                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff, e,
+                            g2);
+                    f0 += -(double)(g2 * dx);
+                    f1 += -(double)(g2 * dy);
+                    f2 += -(double)(g2 * dz);
+                }
+                f0 = wave_sum_dpp(f0);
+                f1 = wave_sum_dpp(f1);
+                f2 = wave_sum_dpp(f2);
+                if (lane == 0) {
+                    s_wA[3 * k] = f0;
+                    s_wA[3 * k + 1] = f1;
+                    s_wA[3 * k + 2] = f2;
+                }
+            }
+            __syncthreads();
+            if (coop > 1) {
+                exchange(s_wA, 3 * w, (2ull * step) & 0x7FFFFull);
+                if (s_abort) break;
+            }
+            // proposals (every lane of slot k's wave forms the same values)
+            if (wave < w) {
+                const int k = wave, b = s_fm0[bw + k];
+                const remc_fb_draws dk = s_fbd[bw + k];
+                const float rx = s_rx[b], ry = s_ry[b], rz = s_rz[b];
+                const float cx = s_bcx[dk.body], cy = s_bcy[dk.body], cz = s_bcz[dk.body];
+                const double F[3] = {s_wA[3 * k], s_wA[3 * k + 1], s_wA[3 * k + 2]};
+                const double tq[3] = {(double)ry * F[2] - (double)rz * F[1], (double)rz * F[0] - (double)rx * F[2],
+                                      (double)rx * F[1] - (double)ry * F[0]};
+                const double A = dk.rotate ? a_r : a_t;
+                const double *fo = dk.rotate ? tq : F;
+                double disp[3], qr[4] = {1.0, 0.0, 0.0, 0.0}, cf[9];
+                for (int t = 0; t < 3; ++t) disp[t] = beta * A * fo[t] + sqrt(2.0 * A) * dk.n[t];
+                float nrx = rx, nry = ry, nrz = rz, ncx = cx, ncy = cy, ncz = cz;
+                if (dk.rotate) {
+                    const double ang = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+                    if (ang > 0.0) {
+                        double sh, ch;
+                        remc_sincos_turns(0.5 * ang / 6.283185307179586476925, &sh, &ch);
+                        qr[0] = ch; qr[1] = sh * disp[0] / ang; qr[2] = sh * disp[1] / ang; qr[3] = sh * disp[2] / ang;
+                        const double qm = sqrt(qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+                        for (int t = 0; t < 4; ++t) qr[t] = qr[t] / qm;
+                    }
+                    remc_qcoeffs(qr, cf);
+                    remc_qrotate(cf, rx, ry, rz, &nrx, &nry, &nrz);
+                } else {
+                    const float t0 = (float)disp[0], t1 = (float)disp[1], t2 = (float)disp[2];
+                    disp[0] = t0; disp[1] = t1; disp[2] = t2; // the displacement actually made
+                    remc_wrap_centre(cx, cy, cz, t0, t1, t2, S.box, &ncx, &ncy, &ncz);
+                }
+                double fwd = 0.0;
+                for (int t = 0; t < 3; ++t) {
+                    const double a = disp[t] - beta * A * fo[t];
+                    fwd += a * a;
+                }
+                if (lane == 0) {
+                    s_wold[k] = make_float4(s_jx[b], s_jy[b], s_jz[b], 0.0f);
+                    s_wnew[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, 0.0f);
+                    s_wnrel[k] = make_float4(nrx, nry, nrz, 0.0f);
+                    s_wncen[k] = make_float4(ncx, ncy, ncz, 0.0f);
+                    for (int t = 0; t < 3; ++t) s_wdisp[3 * k + t] = disp[t];
+                    for (int t = 0; t < 4; ++t) s_wqp[4 * k + t] = qr[t];
+                    s_wfwd[k] = fwd;
+                }
+            }
+            __syncthreads();
+            if (wave < w) {
+                const int k = wave, b = s_fm0[bw + k];
+                // which earlier slots reach slot k (lane j looks at slot j)
+                bool touch = false;
+                if (lane < k) {
+                    if (s_fbd[bw + lane].body == s_fbd[bw + k].body) {
+                        touch = true;
+                    } else {
+                        const float4 oj = s_wold[lane], nj_ = s_wnew[lane], ok = s_wold[k], nk = s_wnew[k];
+                        touch = dist2(oj.x, oj.y, oj.z, ok, S) < ff.crow_cut2 || dist2(oj.x, oj.y, oj.z, nk, S) < ff.crow_cut2 ||
+                                dist2(nj_.x, nj_.y, nj_.z, ok, S) < ff.crow_cut2 ||
+                                dist2(nj_.x, nj_.y, nj_.z, nk, S) < ff.crow_cut2;
+                    }
+                }
+                const unsigned long long tm = __ballot(touch);
+                if (lane == 0) s_wint[k] = (unsigned int)tm;
+                // dE and the force at the new position
+                const float4 o = s_wold[k], n = s_wnew[k];
+                float acc = 0.0f, comp = 0.0f;
+                double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+                for (int j = js + lane; j < je; j += 64) {
+                    if (j == b) continue;
+                    const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
+                    float eo, en, go, g2;
+                    {
+                        const float dx = min_image(o.x - xj, S.box, S.inv_box);
+                        const float dy = min_image(o.y - yj, S.box, S.inv_box);
+                        const float dz = min_image(o.z - zj, S.box, S.inv_box);
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff,
+                                eo, go);
+                    }
+                    const float dx = min_image(n.x - xj, S.box, S.inv_box);
+                    const float dy = min_image(n.y - yj, S.box, S.inv_box);
+                    const float dz = min_image(n.z - zj, S.box, S.inv_box);
+                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff, en,
+                            g2);
+                    kahan_add(acc, comp, en - eo);
+                    f0 += -(double)(g2 * dx);
+                    f1 += -(double)(g2 * dy);
+                    f2 += -(double)(g2 * dz);
+                }
+                const double dE0 = wave_sum_dpp((double)acc - (double)comp);
+                f0 = wave_sum_dpp(f0);
+                f1 = wave_sum_dpp(f1);
+                f2 = wave_sum_dpp(f2);
+                if (lane == 0) {
+                    s_wB[4 * k] = dE0;
+                    s_wB[4 * k + 1] = f0;
+                    s_wB[4 * k + 2] = f1;
+                    s_wB[4 * k + 3] = f2;
+                }
+            }
+            __syncthreads();
+            if (coop > 1) {
+                exchange(s_wB, 4 * w, (2ull * step + 1ull) & 0x7FFFFull);
+                if (s_abort) break;
+            }
+            // decisions (wave 0, lane k = slot k), as one step at a time takes them
+            if (wave == 0) {
+                int dl = 0;
+                double dEk = 0.0;
+                if (lane < w) {
+                    const int k = lane;
+                    const remc_fb_draws dk = s_fbd[bw + k];
+                    const float4 nr = s_wnrel[k];
+                    const double Fn[3] = {s_wB[4 * k + 1], s_wB[4 * k + 2], s_wB[4 * k + 3]};
+                    const double tn[3] = {(double)nr.y * Fn[2] - (double)nr.z * Fn[1],
+                                          (double)nr.z * Fn[0] - (double)nr.x * Fn[2],
+                                          (double)nr.x * Fn[1] - (double)nr.y * Fn[0]};
+                    const double *fn = dk.rotate ? tn : Fn;
+                    const double A = dk.rotate ? a_r : a_t;
+                    double rev = 0.0;
+                    for (int t = 0; t < 3; ++t) {
+                        const double bb = -s_wdisp[3 * k + t] - beta * A * fn[t];
+                        rev += bb * bb;
+                    }
+                    dEk = s_wB[4 * k];
+                    const double x = beta * dEk + (rev - s_wfwd[k]) / (4.0 * A);
+                    dl = remc_metropolis_l(x, 1.0, dk.nlu);
+                }
+                const unsigned long long acc0 = __ballot(dl != 0);
+                const unsigned int il = lane < w ? s_wint[lane] : 0u;
+                const unsigned long long stop = __ballot((il & (unsigned int)acc0) != 0u) | (1ull << w);
+                const int weff = (int)__builtin_ctzll(stop);
+                if (lane < weff) {
+                    s_wdec[lane] = dl;
+                    s_wdE[lane] = dEk;
+                }
+                if (lane == 0) s_weff = weff;
+            }
+            __syncthreads();
+            const int weff = s_weff;
+            // commit the accepted slots (distinct bodies: a repeat ends the window)
+            if (tid < weff && s_wdec[tid]) {
+                const int k = tid, b = s_fm0[bw + k];
+                const remc_fb_draws dk = s_fbd[bw + k];
+                const float4 n = s_wnew[k];
+                gx[b] = n.x;
+                gy[b] = n.y;
+                gz[b] = n.z;
+                s_jx[b] = n.x;
+                s_jy[b] = n.y;
+                s_jz[b] = n.z;
+                if (dk.rotate) {
+                    const float4 nr = s_wnrel[k];
+                    grx[b] = nr.x;
+                    gry[b] = nr.y;
+                    grz[b] = nr.z;
+                    s_rx[b] = nr.x;
+                    s_ry[b] = nr.y;
+                    s_rz[b] = nr.z;
+                    double qp[4], qo[4], qn[4];
+                    for (int t = 0; t < 4; ++t) {
+                        qp[t] = s_wqp[4 * k + t];
+                        qo[t] = s_bq[4 * dk.body + t];
+                    }
+                    remc_qmul(qp, qo, qn);
+                    for (int t = 0; t < 4; ++t) {
+                        gq[4 * dk.body + t] = qn[t];
+                        s_bq[4 * dk.body + t] = qn[t];
+                    }
+                } else {
+                    const float4 nc = s_wncen[k];
+                    gcx[dk.body] = nc.x;
+                    gcy[dk.body] = nc.y;
+                    gcz[dk.body] = nc.z;
+                    s_bcx[dk.body] = nc.x;
+                    s_bcy[dk.body] = nc.y;
+                    s_bcz[dk.body] = nc.z;
+                }
+            }
+            if (c == 0 && p.rec && tid < weff) {
+                const unsigned long long st = step + tid;
+                const unsigned long long rec_start = p.rec_start[r];
+                const unsigned long long idx = st - rec_start;
+                if (st >= rec_start && idx < (unsigned long long)p.rec_max) {
+                    remc_step_record rr;
+                    rr.body = s_fbd[bw + tid].body;
+                    rr.move = s_fbd[bw + tid].rotate ? 1 : 0;
+                    rr.accepted = s_wdec[tid];
+                    rr.ticks = (int32_t)((wall_clock64() - t_step) / (unsigned long long)weff);
+                    rr.delta = s_wdE[tid];
+                    p.rec[(size_t)r * p.rec_max + idx] = rr;
+                }
+            }
+            // the tracked energy and the counters, in step order, in every thread
+            for (int k = 0; k < weff; ++k) {
+                const int dk = s_wdec[k];
+                if (dk) {
+                    E += s_wdE[k];
+                    if (dk == 1) ++n_acc; else ++n_accb;
+                } else {
+                    ++n_rej;
+                }
+            }
+            __syncthreads();
+            it += weff - 1;
+            step += weff - 1;
+            continue;
         }
         const remc_fb_draws d = s_fbd[it & 63];
         const int body = d.body;
-        const int m0 = p.bstart[S.body_off + body], m1 = p.bend[S.body_off + body], nm = m1 - m0;
-        const bool crow_m = p.bcrow[S.body_off + body] != 0;
+        const int m0 = s_fm0[it & 63], nm = s_fnm[it & 63], m1 = m0 + nm;
+        const bool crow_m = s_fcrow[it & 63] != 0;
+        // the body's centre, needed by the proposal after pass A
+        const float cx0 = cache_bodies ? s_bcx[body] : gcx[body], cy0 = cache_bodies ? s_bcy[body] : gcy[body],
+                    cz0 = cache_bodies ? s_bcz[body] : gcz[body];
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
             s_old[k] = cached ? make_float4(s_jx[b], s_jy[b], s_jz[b], s_jq[b] * ff.c_dh)
                               : make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
-            s_orel[k] = make_float4(grx[b], gry[b], grz[b], sgs[b]);
-            s_tc[k] = tcs[b];
+            s_orel[k] = cache_rel ? make_float4(s_rx[b], s_ry[b], s_rz[b], s_js[b])
+                                  : make_float4(grx[b], gry[b], grz[b], sgs[b]);
+            s_tc[k] = cached ? s_jtc[b] : tcs[b];
         }
         __syncthreads();
 
+        // A large move is split over the members' partner slices [cjs, cje),
+        // the partial forces, torques and dE then exchanged; a small one is
+        // evaluated whole by every member, with the same result in each.
+        const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
+        const int js = split ? cjs : 0, je = split ? cje : N;
+        const int ovs = max(js, m0), ov = max(0, min(je, m1) - ovs);
+        const int nj = (je - js) - ov;
+
         // pass A: force and torque at the old position; the (moved bead x
         // partner bead) pairs split over the workgroup as in the MC kernel
-        const int nj = N - nm;
         int G, kper;
         pick_split<BLOCK>(nm, nj, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
         for (int jj = l; jj < nj; jj += G) {
-            const int j = jj + (jj >= m0 ? nm : 0);
+            const int j = js + jj + (js + jj >= ovs ? ov : 0);
             const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
             const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
             const int tcj = cached ? s_jtc[j] : tcs[j];
@@ -1232,45 +1608,53 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
             }
         }
         block_sum(v, 6);
-
-        // proposal (thread 0)
-        if (tid == 0) {
-            const double A = d.rotate ? a_r : a_t;
-            const double *fo = d.rotate ? s_ft + 3 : s_ft;
-            double disp[3];
-            for (int k = 0; k < 3; ++k) disp[k] = beta * A * fo[k] + sqrt(2.0 * A) * d.n[k];
-            const float cx = gcx[body], cy = gcy[body], cz = gcz[body];
-            if (d.rotate) {
-                const double ang = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
-                double q[4] = {1.0, 0.0, 0.0, 0.0};
-                if (ang > 0.0) {
-                    double sh, ch;
-                    remc_sincos_turns(0.5 * ang / 6.283185307179586476925, &sh, &ch);
-                    q[0] = ch; q[1] = sh * disp[0] / ang; q[2] = sh * disp[1] / ang; q[3] = sh * disp[2] / ang;
-                    const double qm = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-                    for (int k = 0; k < 4; ++k) q[k] = q[k] / qm;
-                }
-                double cf[9];
-                remc_qcoeffs(q, cf);
-                for (int k = 0; k < 4; ++k) s_q[k] = q[k];
-                for (int k = 0; k < 9; ++k) s_coef[k] = cf[k];
-                s_cen[0] = cx; s_cen[1] = cy; s_cen[2] = cz;
-            } else {
-                const float t0 = (float)disp[0], t1 = (float)disp[1], t2 = (float)disp[2];
-                disp[0] = t0; disp[1] = t1; disp[2] = t2; // the displacement actually made
-                float px, py, pz;
-                remc_wrap_centre(cx, cy, cz, t0, t1, t2, S.box, &px, &py, &pz);
-                s_cen[0] = px; s_cen[1] = py; s_cen[2] = pz;
-            }
-            for (int k = 0; k < 3; ++k) s_disp[k] = disp[k];
+        if (split) {
+            exchange(s_ft, 6, (2ull * step) & 0x7FFFFull);
+            if (s_abort) break;
         }
-        __syncthreads();
+
+        // proposal, formed by every thread from the same totals (identical
+        // values everywhere, no broadcast): the displacement or rotation, the
+        // new centre, and for a rotation its quaternion and coefficients
+        const double A = d.rotate ? a_r : a_t;
+        double fo[3], disp[3], qr[4] = {1.0, 0.0, 0.0, 0.0}, cf[9];
+        float ncx = cx0, ncy = cy0, ncz = cz0;
+        for (int k = 0; k < 3; ++k) fo[k] = s_ft[(d.rotate ? 3 : 0) + k];
+        for (int k = 0; k < 3; ++k) disp[k] = beta * A * fo[k] + sqrt(2.0 * A) * d.n[k];
+        if (d.rotate) {
+            const double ang = sqrt(disp[0] * disp[0] + disp[1] * disp[1] + disp[2] * disp[2]);
+            if (ang > 0.0) {
+                double sh, ch;
+                remc_sincos_turns(0.5 * ang / 6.283185307179586476925, &sh, &ch);
+                qr[0] = ch; qr[1] = sh * disp[0] / ang; qr[2] = sh * disp[1] / ang; qr[3] = sh * disp[2] / ang;
+                const double qm = sqrt(qr[0] * qr[0] + qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+                for (int k = 0; k < 4; ++k) qr[k] = qr[k] / qm;
+            }
+            remc_qcoeffs(qr, cf);
+        } else {
+            const float t0 = (float)disp[0], t1 = (float)disp[1], t2 = (float)disp[2];
+            disp[0] = t0; disp[1] = t1; disp[2] = t2; // the displacement actually made
+            remc_wrap_centre(cx0, cy0, cz0, t0, t1, t2, S.box, &ncx, &ncy, &ncz);
+        }
         for (int k = tid; k < nm; k += BLOCK) {
             const float4 orl = s_orel[k];
             float nrx = orl.x, nry = orl.y, nrz = orl.z;
-            if (d.rotate) remc_qrotate(s_coef, orl.x, orl.y, orl.z, &nrx, &nry, &nrz);
+            if (d.rotate) remc_qrotate(cf, orl.x, orl.y, orl.z, &nrx, &nry, &nrz);
             s_nrel[k] = make_float4(nrx, nry, nrz, 0.0f);
-            s_new[k] = make_float4(nrx + s_cen[0], nry + s_cen[1], nrz + s_cen[2], 0.0f);
+            s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, 0.0f);
+        }
+        // what thread 0 commits (the other threads need neither after this)
+        if (tid == 0) {
+            for (int k = 0; k < 4; ++k) s_qr[k] = qr[k];
+            s_ncen[0] = ncx;
+            s_ncen[1] = ncy;
+            s_ncen[2] = ncz;
+        }
+        // the forward half of the proposal ratio
+        double fwd = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double a = disp[k] - beta * A * fo[k];
+            fwd += a * a;
         }
         __syncthreads();
 
@@ -1278,7 +1662,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         float acc = 0.0f, comp = 0.0f;
         for (int k = 0; k < 6; ++k) v[k] = 0.0;
         for (int jj = l; jj < nj; jj += G) {
-            const int j = jj + (jj >= m0 ? nm : 0);
+            const int j = js + jj + (js + jj >= ovs ? ov : 0);
             const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
             const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
             const int tcj = cached ? s_jtc[j] : tcs[j];
@@ -1310,30 +1694,20 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
             }
         }
         v[6] = (double)acc - (double)comp;
-        // the old force and torque are needed again after the reduction reuses s_ft
-        double fo[3] = {0.0, 0.0, 0.0};
-        if (tid == 0) {
-            const int o3 = d.rotate ? 3 : 0;
-            fo[0] = s_ft[o3]; fo[1] = s_ft[o3 + 1]; fo[2] = s_ft[o3 + 2];
-        }
         block_sum(v, 7);
-        if (tid == 0) {
-            const double A = d.rotate ? a_r : a_t;
-            const double *fn = d.rotate ? s_ft + 3 : s_ft;
-            double fwd = 0.0, rev = 0.0;
-            for (int k = 0; k < 3; ++k) {
-                const double a = s_disp[k] - beta * A * fo[k], b = -s_disp[k] - beta * A * fn[k];
-                fwd += a * a;
-                rev += b * b;
-            }
-            const double dE = s_ft[6];
-            const double x = beta * dE + (rev - fwd) / (4.0 * A);
-            s_dec = remc_metropolis_l(x, 1.0, d.nlu);
-            s_dE = dE;
+        if (split) {
+            exchange(s_ft, 7, (2ull * step + 1ull) & 0x7FFFFull);
+            if (s_abort) break;
         }
-        __syncthreads();
-        const int dec = s_dec;
-        const double dE = s_dE;
+        // the decision, taken by every thread from the same totals
+        double rev = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double b = -disp[k] - beta * A * s_ft[(d.rotate ? 3 : 0) + k];
+            rev += b * b;
+        }
+        const double dE = s_ft[6];
+        const double x = beta * dE + (rev - fwd) / (4.0 * A);
+        const int dec = remc_metropolis_l(x, 1.0, d.nlu);
         if (dec) {
             for (int k = tid; k < nm; k += BLOCK) {
                 const int b = m0 + k;
@@ -1351,17 +1725,31 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
                     grx[b] = nrl.x;
                     gry[b] = nrl.y;
                     grz[b] = nrl.z;
+                    if (cache_rel) {
+                        s_rx[b] = nrl.x;
+                        s_ry[b] = nrl.y;
+                        s_rz[b] = nrl.z;
+                    }
                 }
             }
             if (tid == 0) {
                 if (d.rotate) {
-                    double qo[4] = {gq[4 * body], gq[4 * body + 1], gq[4 * body + 2], gq[4 * body + 3]}, qn[4];
-                    remc_qmul(s_q, qo, qn);
-                    for (int k = 0; k < 4; ++k) gq[4 * body + k] = qn[k];
+                    double qo[4], qn[4];
+                    for (int k = 0; k < 4; ++k) qo[k] = cache_bodies ? s_bq[4 * body + k] : gq[4 * body + k];
+                    remc_qmul(s_qr, qo, qn);
+                    for (int k = 0; k < 4; ++k) {
+                        gq[4 * body + k] = qn[k];
+                        if (cache_bodies) s_bq[4 * body + k] = qn[k];
+                    }
                 } else {
-                    gcx[body] = s_cen[0];
-                    gcy[body] = s_cen[1];
-                    gcz[body] = s_cen[2];
+                    gcx[body] = s_ncen[0];
+                    gcy[body] = s_ncen[1];
+                    gcz[body] = s_ncen[2];
+                    if (cache_bodies) {
+                        s_bcx[body] = s_ncen[0];
+                        s_bcy[body] = s_ncen[1];
+                        s_bcz[body] = s_ncen[2];
+                    }
                 }
             }
             E += dE;
@@ -1369,7 +1757,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         } else {
             ++n_rej;
         }
-        if (p.rec && tid == 0) {
+        if (c == 0 && p.rec && tid == 0) {
             const unsigned long long rec_start = p.rec_start[r];
             const unsigned long long idx = step - rec_start;
             if (step >= rec_start && idx < (unsigned long long)p.rec_max) {
@@ -1384,7 +1772,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
         }
         __syncthreads();
     }
-    if (tid == 0) {
+    if (c == 0 && tid == 0 && !s_abort) {
         p.E[r] = E;
         p.step[r] = step;
         p.cnt[3 * r + 0] += n_acc;
@@ -1395,26 +1783,66 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps)
 
 size_t mc_fb_dynamic_lds(const KParams &p)
 {
-    return (4 * sizeof(float4) + sizeof(int)) * (size_t)p.lds_mb + 24 * (size_t)p.lds_cl;
+    return (4 * sizeof(float4) + sizeof(int)) * (size_t)p.lds_mb +
+           (p.lds_cl > 0 ? 36 * (size_t)p.lds_cl + 8 + 44 * (size_t)p.lds_nbody : 0);
 }
 
-template <int BLOCK>
-static hipError_t launch_mc_fb_t(const KParams &p, int n_rep, int nsteps, hipStream_t s)
+template <int BLOCK, bool CACHE>
+static hipError_t launch_mc_fb_t(const KParams &p, int n_rep, int nsteps, int coop, hipStream_t s)
 {
     const size_t lds = mc_fb_dynamic_lds(p);
-    hipError_t e = hipFuncSetAttribute((const void *)k_mc_fb<BLOCK>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipError_t e = hipFuncSetAttribute((const void *)k_mc_fb<BLOCK, CACHE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_mc_fb<BLOCK>), dim3(n_rep), dim3(BLOCK), lds, s, p, nsteps);
+    hipLaunchKernelGGL((k_mc_fb<BLOCK, CACHE>), dim3(n_rep * coop), dim3(BLOCK), lds, s, p, nsteps, coop);
     return hipGetLastError();
 }
 
-hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, hipStream_t s)
+template <int BLOCK, bool CACHE>
+static hipError_t fb_occupancy_t(const KParams &p, int *out)
+{
+    const size_t lds = mc_fb_dynamic_lds(p);
+    const void *f = (const void *)k_mc_fb<BLOCK, CACHE>;
+    hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(out, k_mc_fb<BLOCK, CACHE>, BLOCK, lds);
+}
+
+template <int BLOCK>
+static hipError_t fb_occupancy_b(const KParams &p, int *out)
+{
+    return p.lds_cl > 0 ? fb_occupancy_t<BLOCK, true>(p, out) : fb_occupancy_t<BLOCK, false>(p, out);
+}
+
+hipError_t mc_fb_max_active_per_cu(const KParams &p, int block, int *out)
+{
+    switch (block) {
+    case 256: return fb_occupancy_b<256>(p, out);
+    case 512: return fb_occupancy_b<512>(p, out);
+    case 1024: return fb_occupancy_b<1024>(p, out);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int BLOCK>
+static hipError_t launch_mc_fb_b(const KParams &p, int n_rep, int nsteps, int coop, hipStream_t s)
+{
+    return p.lds_cl > 0 ? launch_mc_fb_t<BLOCK, true>(p, n_rep, nsteps, coop, s)
+                        : launch_mc_fb_t<BLOCK, false>(p, n_rep, nsteps, coop, s);
+}
+
+hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s)
 {
     if (n_rep <= 0 || nsteps <= 0) return hipSuccess;
+    if (coop < 1 || coop > REMC_MAX_COOP) return hipErrorInvalidValue;
+    if (coop > 1) {
+        const int n = n_rep * REMC_SLOTS_PER_REP;
+        hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
+    }
     switch (block) {
-    case 256: return launch_mc_fb_t<256>(p, n_rep, nsteps, s);
-    case 512: return launch_mc_fb_t<512>(p, n_rep, nsteps, s);
-    case 1024: return launch_mc_fb_t<1024>(p, n_rep, nsteps, s);
+    case 256: return launch_mc_fb_b<256>(p, n_rep, nsteps, coop, s);
+    case 512: return launch_mc_fb_b<512>(p, n_rep, nsteps, coop, s);
+    case 1024: return launch_mc_fb_b<1024>(p, n_rep, nsteps, coop, s);
     default: return hipErrorInvalidValue;
     }
 }

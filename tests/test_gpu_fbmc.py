@@ -53,12 +53,14 @@ def compare_fb(grec, orec, T, guard=1e-3, delta_tol_kT=1e-4):
     return len(orec), False, kinds
 
 
-def run_fb_parity(oracle, cid, nsteps, replicas, block=0):
+def run_fb_parity(oracle, cid, nsteps, replicas, block=0, coop=None):
     wl = synth.config(cid)
     sa = wl.systems[0]
     lad = wl.temperatures[0]
     a_t, a_r = mobilities(wl)
     e = engine_for(wl, seed=SEED)
+    if coop:
+        e.set_mc_coop(coop)
     e.set_recording(nsteps)
     e.mc_steps_fb(nsteps, block)
     e.synchronize()
@@ -102,6 +104,14 @@ def test_fb_parity_two_chains(oracle):
 @pytest.mark.parametrize("block", [256, 1024])
 def test_fb_parity_chains_and_crowders(oracle, block):
     assert run_fb_parity(oracle, 2, 800, [0, 7, 19], block=block) <= 1
+
+
+@pytest.mark.parametrize("coop", [1, 8])
+def test_fb_parity_large_moves_split_over_workgroups(oracle, coop):
+    """Config 4: a chain move's forces, torques and dE are split over 8
+    workgroups per replica (partner slices, partials exchanged twice per step)
+    or taken by one; crowder moves are evaluated whole by every member."""
+    assert run_fb_parity(oracle, 4, 400, [0, 13, 31], coop=coop) <= 1
 
 
 def test_fb_rejects_bad_block():
