@@ -136,9 +136,9 @@ REMC_API int remc_refresh_energies(remc_ctx *ctx);
 REMC_API int remc_energy_noncrowder_all(remc_ctx *ctx, double *out);
 
 /* ------------------------------------------------------------------ MC --- */
-/* nsteps MC steps on every replica in one persistent launch (one workgroup
- * per replica): device-side move proposal, dE of the moved body, Metropolis
- * in FP64.  Replaces the MCSearch loop (Replica.cpp:368-478) with its per-step
+/* nsteps MC steps on every replica in one persistent launch (one or more
+ * cooperating workgroups per replica, remc_set_mc_coop): device-side move
+ * proposal, dE of the moved body, Metropolis in FP64.  Replaces the MCSearch loop (Replica.cpp:368-478) with its per-step
  * MoleculeDataToDevice + EonDevice round trips. */
 REMC_API int remc_mc_steps(remc_ctx *ctx, int nsteps);
 /* The reference protocol for one replica: host proposal, host->device copy
@@ -146,8 +146,8 @@ REMC_API int remc_mc_steps(remc_ctx *ctx, int nsteps);
  * Metropolis, copy-back on reject (Replica.cpp:386-474).  Uses the same RNG
  * stream and move arithmetic as remc_mc_steps. */
 REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int block);
-/* nsteps force-biased (smart) MC steps on every replica, one workgroup per
- * replica (Rossky, Doll & Friedman, J. Chem. Phys. 69, 4628 (1978)): the
+/* nsteps force-biased (smart) MC steps on every replica (Rossky, Doll &
+ * Friedman, J. Chem. Phys. 69, 4628 (1978)), one persistent launch: the
  * chosen body's translation is drawn as d = beta A F + sqrt(2A) n, or its
  * rotation vector as beta A tau + sqrt(2A) n, from the force F on the body or
  * the torque tau about its centre (n standard normal; A = t^2/6 for
@@ -156,8 +156,13 @@ REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int blo
  * keeps exp(-beta E).  No counterpart in the reference: it replaces the
  * uniform proposal of Replica::MCSearch (Replica.cpp:386-420) with one driven
  * by the gradient that remc_adjoint returns.  Shares the replicas, energies,
- * step and decision counters with remc_mc_steps.  block: workgroup size (256,
- * 512 or 1024; 0 = 512). */
+ * step and decision counters with remc_mc_steps.  A replica runs on the
+ * workgroups per replica set by remc_set_mc_coop (or the MC autotuner), else
+ * on as many as give each its own CU, at most what runs co-resident; a large
+ * move is split over them.  Runs of one-bead crowder moves are taken in
+ * windows (one per wave) when the replica fits in LDS, switched with
+ * remc_set_mc_window; decisions and states equal those of one step at a time
+ * up to rounding.  block: workgroup size (256, 512 or 1024; 0 = 512). */
 REMC_API int remc_mc_steps_fb(remc_ctx *ctx, int nsteps, int block);
 /* One replica-exchange round over every system's temperature ladder, pairs
  * (k,k+1) for k = offset, offset+2, ...; the offset alternates per call
@@ -250,7 +255,7 @@ REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
  * LDS cache; split_min_pairs (> 0) is the pair-term count from which a move is
  * split across cooperating workgroups (default 8192; 0 keeps the current). */
 REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
-/* Speculative windows of the MC kernel (default on): up to two steps per wave
+/* Speculative windows of the MC kernels (default on): up to two steps per wave
  * (32 with 1024-thread workgroups) of consecutive moves of small bodies (at
  * most 16 beads) are evaluated at once against the window's starting
  * conformation (with cooperating workgroups, each over its partner slice); the
@@ -260,8 +265,9 @@ REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pair
  * window moved is proposed from that step's proposal instead, and ends the
  * window only if that step is rejected.  Used when a partner cache fits in
  * LDS (the replica, or each member's slice).  Decisions and states equal those
- * of one step at a time up to FP32 rounding.  0 turns the windows off.  No
- * counterpart in the reference. */
+ * of one step at a time up to FP32 rounding.  The force-biased kernel's windows
+ * of one-bead crowder moves follow the same switch.  0 turns the windows off.
+ * No counterpart in the reference. */
 REMC_API int remc_set_mc_window(remc_ctx *ctx, int enable);
 
 #ifdef __cplusplus
