@@ -12,6 +12,10 @@ x = 0 to within rounding (the proposal ratio cancels beta dE to third order
 in the step), so the kind of an acceptance may differ there while the state
 stays the same; the comparison goes on past such steps.
 
+Runs of one-bead crowder moves are taken in windows (one slot per wave,
+exact by the crowder cutoff); the same parity bars hold with the windows on
+and off, with one workgroup per replica and with several (large moves split).
+
 Equilibrium: two oppositely charged beads attracting by contact in a
 periodic box, at one temperature, sampled by many replicas at once.  For a
 separation r < L/2 the exact density is r^2 exp(-beta U(r)) (the relative
@@ -53,7 +57,7 @@ def compare_fb(grec, orec, T, guard=1e-3, delta_tol_kT=1e-4):
     return len(orec), False, kinds
 
 
-def run_fb_parity(oracle, cid, nsteps, replicas, block=0, coop=None):
+def run_fb_parity(oracle, cid, nsteps, replicas, block=0, coop=None, window=True):
     wl = synth.config(cid)
     sa = wl.systems[0]
     lad = wl.temperatures[0]
@@ -61,6 +65,8 @@ def run_fb_parity(oracle, cid, nsteps, replicas, block=0, coop=None):
     e = engine_for(wl, seed=SEED)
     if coop:
         e.set_mc_coop(coop)
+    if not window:
+        e.set_mc_window(False)
     e.set_recording(nsteps)
     e.mc_steps_fb(nsteps, block)
     e.synchronize()
@@ -112,6 +118,12 @@ def test_fb_parity_large_moves_split_over_workgroups(oracle, coop):
     workgroups per replica (partner slices, partials exchanged twice per step)
     or taken by one; crowder moves are evaluated whole by every member."""
     assert run_fb_parity(oracle, 4, 400, [0, 13, 31], coop=coop) <= 1
+
+
+@pytest.mark.parametrize("cid", [2, 4])
+def test_fb_parity_windows_off(oracle, cid):
+    """Every crowder move taken one step at a time (the windows switched off)."""
+    assert run_fb_parity(oracle, cid, 400, [0, 7, 19], coop=4, window=False) <= 1
 
 
 def test_fb_rejects_bad_block():
