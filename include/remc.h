@@ -255,8 +255,8 @@ REMC_API int remc_get_mc_coop(remc_ctx *ctx, int *coop);
  * LDS cache; split_min_pairs (> 0) is the pair-term count from which a move is
  * split across cooperating workgroups (default 8192; 0 keeps the current). */
 REMC_API int remc_set_mc_tuning(remc_ctx *ctx, int lds_cache, int split_min_pairs);
-/* Speculative windows of the MC kernels (default on): up to 32 steps (at most
- * four per wave, as the LDS left after the caches allows) of consecutive moves of small bodies (at
+/* Speculative windows of the MC kernels (default on): up to two steps per wave
+ * (32 with 1024-thread workgroups) of consecutive moves of small bodies (at
  * most 16 beads) are evaluated at once against the window's starting
  * conformation (with cooperating workgroups, each over its partner slice); the
  * window ends before the first step that an accepted earlier step of the

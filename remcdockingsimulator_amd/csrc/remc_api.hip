@@ -500,25 +500,10 @@ int check_err(remc_ctx *c)
     return REMC_OK;
 }
 
-// Speculative windows (they read the partner range, the replica or the
-// member's slice, from LDS): the most slots, up to four per wave and
-// MC_WINDOW_MAX, whose staging fits the LDS left after the caches.
-void set_window(KParams &p, bool window, int block, long long budget)
-{
-    if (!window) return;
-    const int nw = block / 64;
-    for (int wk = std::min(4 * nw, MC_WINDOW_MAX); wk >= nw; wk /= 2)
-        if (16 + 52LL * wk * SPEC_MAX <= budget) {
-            p.spec = 1;
-            p.wk = wk;
-            return;
-        }
-}
-
 // LDS layout of the MC kernel for a given cooperation: the moved-body staging
 // always; the partner beads of the replica (or of the member's slice when
 // every move is split) and the body table when they fit the LDS share of one
-// workgroup at the occupancy the grid needs; the window slots in what is left.
+// workgroup at the occupancy the grid needs.
 void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
 {
     const int R = (int)c->reps.size();
@@ -537,7 +522,6 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
     p.cache_rel = 0;
     p.cache_quat = 0;
     p.spec = 0;
-    p.wk = 0;
     if (!c->lds_cache || budget <= 0) return;
     const long long full = 24LL * c->max_beads;
     const bool all_split = coop > 1 && c->min_move_pairs >= c->coop_min_pairs;
@@ -552,13 +536,13 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
     } else {
         return;
     }
-    const bool window = c->mc_window != 0;
+    // speculative windows read the partner range (the replica, or the member's slice) from LDS
+    p.spec = c->mc_window ? 1 : 0;
     if (20LL * c->max_bodies <= budget) {
         p.lds_nbody = c->max_bodies;
         p.cache_bodies = 1;
         budget -= 20LL * c->max_bodies;
     } else {
-        set_window(p, window, block, budget);
         return;
     }
     if (!p.cache_slice && 12LL * c->max_beads <= budget) {
@@ -569,7 +553,6 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
         p.cache_quat = 1;
         budget -= 32LL * c->max_bodies;
     }
-    set_window(p, window, block, budget);
 }
 
 // Largest workgroups-per-replica count whose grid is co-resident.

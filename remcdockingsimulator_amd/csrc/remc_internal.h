@@ -91,7 +91,6 @@ struct KParams {
     int cache_rel;          // 1: relative positions cached (full partner cache only)
     int cache_quat;         // 1: body rotations cached (with the body table)
     int spec;               // 1: speculative windows of small moves (needs the full caches)
-    int wk;                 // window slots (a power of two, a multiple of the waves, wk * SPEC_MAX <= block)
     // recording
     remc_step_record *rec;
     unsigned long long *rec_start;
@@ -125,11 +124,6 @@ void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, do
 constexpr int FULL_TILE = 256;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
-// most slots of a speculative window: 64-slot windows (four per wave of a
-// 1024-thread workgroup) measured slower than 32 on the crowded workloads, the
-// extra slots' pair corrections and interaction stops costing more than the
-// windows they save
-constexpr int MC_WINDOW_MAX = 32;
 constexpr int REMC_SLOT_K = 32;  // partial sums per member and exchange (window slots; 1 for a split step)
 // two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica
 constexpr int REMC_SLOTS_PER_REP = 2 * REMC_SLOT_K * REMC_MAX_COOP;

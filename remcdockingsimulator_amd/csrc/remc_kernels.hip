@@ -207,7 +207,6 @@ __device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, c
 //   CACHE:                  float x,y,z,q,sig[cl]; int tc[cl]  partner beads [clo, clo+cl)
 //   CACHE && cache_rel:     float rx,ry,rz[cl]                 relative positions
 //   CACHE && cache_bodies:  int start,end[nbody]; float cx,cy,cz[nbody]
-//   spec (16-byte aligned): float4 old,new,rel[wk*SPEC_MAX]; int tc[wk*SPEC_MAX]  window slots
 // mb, cl and nbody are the launch-wide maxima (KParams::lds_*).
 size_t mc_dynamic_lds(const KParams &p)
 {
@@ -217,16 +216,15 @@ size_t mc_dynamic_lds(const KParams &p)
     if (cl) s += 24 * cl;
     if (cl && p.cache_rel) s += 12 * cl;
     if (cl && p.cache_bodies) s += 20 * nb;
-    if (p.spec) s += 16 + 52 * (size_t)p.wk * SPEC_MAX;
     return s;
 }
 
 // One launch runs nsteps MC steps of every replica.  A step whose moved body
 // is small (at most SPEC_MAX beads) may be taken in a speculative window when
-// SPEC is set: the next w <= WK such steps are evaluated at once, up to 32
-// slots, at most four per wave (with cooperating members, each over its partner
-// slice, the partials exchanged once per window), each against the
-// conformation at the window's start (dE0_k).  The exact dE of step k after the accepted steps j < k of the
+// SPEC is set: the next w <= WK such steps are evaluated at once, two slots per
+// wave (with cooperating members, each over its partner slice, the partials
+// exchanged once per window), each against the conformation at the window's
+// start (dE0_k).  The exact dE of step k after the accepted steps j < k of the
 // window differs from dE0_k only by the pair terms between the two moved
 // bodies,
 //   corr(j,k) = sum_{i in k, l in j} [e(n_i,n_l) - e(o_i,n_l)] - [e(n_i,o_l) - e(o_i,o_l)],
@@ -242,27 +240,26 @@ template <int BLOCK, bool CACHE, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 {
     constexpr int NW = BLOCK / 64;
-    // window slots of this launch (p.wk, a power of two, a multiple of NW with
-    // wk * SPEC_MAX <= BLOCK) and their largest count
-    constexpr int WKM = SPEC ? MC_WINDOW_MAX : 1;
-    static_assert(WKM <= REMC_SLOT_K, "window slots");
-    const int WK = SPEC ? p.wk : 1;
-    const int WS = WK * SPEC_MAX;
+    constexpr int WK = SPEC ? 2 * NW : 1; // window slots: at most 32 (the slot masks are 32-bit)
+    static_assert(WK <= 32 && WK <= REMC_SLOT_K, "window slots");
+    constexpr int WS = SPEC ? WK * SPEC_MAX : 1;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
     __shared__ StepDraw s_draw[DRAW_BATCH];
     __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
-    __shared__ float4 s_wcen[WKM];                        // new centre of a translated slot
-    __shared__ double s_wq[4 * WKM];                      // new rotation of a rotated slot
-    __shared__ double s_wbase[WKM], s_wdE[WKM];
-    __shared__ unsigned long long s_wint[WKM]; // bit j of slot k: slot j's move changes slot k's dE
+    __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old (w: q c_dh) / new (w: sigma) position, new rel (w: q)
+    __shared__ int s_wtc[WS];                             // type | crowder << 8 of the staged beads
+    __shared__ float4 s_wcen[WK];                         // new centre of a translated slot
+    __shared__ double s_wq[4 * WK];                       // new rotation of a rotated slot
+    __shared__ double s_wbase[WK], s_wdE[WK];
+    __shared__ unsigned int s_wint[WK]; // bit j of slot k: slot j's move changes slot k's dE
     // bit j of slot k: slot k moves the body slot j moved (j the only earlier
     // such slot), and is staged from slot j's proposal
-    __shared__ unsigned long long s_wchain[WKM];
-    __shared__ int s_wnext[WKM]; // the accepted slot chained to slot j, or -1
-    __shared__ int s_wdec[WKM];
+    __shared__ unsigned int s_wchain[WK];
+    __shared__ int s_wnext[WK]; // the accepted slot chained to slot j, or -1
+    __shared__ int s_wdec[WK];
     __shared__ int s_weff;
     __shared__ double s_E;
     __shared__ unsigned long long s_cnt[3];
@@ -281,11 +278,6 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     float *s_rx = (float *)(s_jtc + p.lds_cl), *s_ry = s_rx + rel_n, *s_rz = s_ry + rel_n;
     int *s_bs = (int *)(s_rz + rel_n), *s_be = s_bs + p.lds_nbody;
     float *s_cx = (float *)(s_be + p.lds_nbody), *s_cy = s_cx + p.lds_nbody, *s_cz = s_cy + p.lds_nbody;
-    // window slot staging: old (w: q c_dh) / new (w: sigma) position, new rel (w: q), type | crowder << 8
-    char *const w_at = (char *)(s_cz + p.lds_nbody);
-    float4 *s_wold = (float4 *)(w_at + ((16 - ((w_at - (char *)smem) & 15)) & 15));
-    float4 *s_wnew = s_wold + WS, *s_wrel = s_wnew + WS;
-    int *s_wtc = (int *)(s_wrel + WS);
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -374,7 +366,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             // fewer than WK draws left: form the next batch now if the run of
             // window-eligible steps from here reaches the batch's end (else a
             // large move ends the window before it anyway)
-            const unsigned long long rest = (1ull << (have - off)) - 1ull; // have - off < WK <= 64
+            const unsigned long long rest = (1ull << (have - off)) - 1ull; // have - off < WK <= 32
             refill = ((~(elig >> off)) & rest) == 0ull;
         }
         if (refill) {
@@ -445,7 +437,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
             // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
-            if (tid < WK) s_wint[tid] = 0ull;
+            if (tid < WK) s_wint[tid] = 0u;
             if (tid < WK * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < w) {
@@ -459,7 +451,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     const int pj = Dk.prev1 - off;
                     const bool chained = pj >= 0 && Dk.prev2 < off;
                     if (i == 0) {
-                        s_wchain[k] = chained ? 1ull << pj : 0ull;
+                        s_wchain[k] = chained ? 1u << pj : 0u;
                         s_wnext[k] = -1;
                     }
                     if (i < nmk) {
@@ -517,93 +509,90 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             const unsigned long long wtag = step & 0x7FFFFull;
             unsigned long long *bank = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
             const int wjs = coop > 1 ? cjs : 0, wje = coop > 1 ? cje : N;
-            for (int ka = wave; ka < w; ka += 2 * NW) {
-                const int kb = ka + NW;
-                if (kb < w && s_draw[off + ka].nm == 1 && s_draw[off + kb].nm == 1 &&
-                    ((s_wtc[ka * SPEC_MAX] & s_wtc[kb * SPEC_MAX]) >> 8) != 0) {
-                    // both slots of the wave move one crowder bead: one pass over the
-                    // partners serves both (the same per-lane order of terms as below)
-                    const float4 oa = s_wold[ka * SPEC_MAX], na = s_wnew[ka * SPEC_MAX];
-                    const float4 ob = s_wold[kb * SPEC_MAX], nb = s_wnew[kb * SPEC_MAX];
-                    const int ba = s_draw[off + ka].m0, bb = s_draw[off + kb].m0;
-                    float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
-                    for (int j = wjs + lane; j < wje; j += 64) {
-                        const int jc = j - clo;
-                        const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
-                        if (j != ba) {
-                            const float eo = pair_crowder(dist2(xj, yj, zj, oa, S), ff);
-                            const float en = pair_crowder(dist2(xj, yj, zj, na, S), ff);
-                            kahan_add(acc_a, comp_a, en - eo);
-                        }
-                        if (j != bb) {
-                            const float eo = pair_crowder(dist2(xj, yj, zj, ob, S), ff);
-                            const float en = pair_crowder(dist2(xj, yj, zj, nb, S), ff);
-                            kahan_add(acc_b, comp_b, en - eo);
-                        }
+            int k0 = wave;
+            if (wave + NW < w && s_draw[off + wave].nm == 1 && s_draw[off + wave + NW].nm == 1 &&
+                ((s_wtc[wave * SPEC_MAX] & s_wtc[(wave + NW) * SPEC_MAX]) >> 8) != 0) {
+                // both slots of the wave move one crowder bead: one pass over the
+                // partners serves both (the same per-lane order of terms as below)
+                const int ka = wave, kb = wave + NW;
+                const float4 oa = s_wold[ka * SPEC_MAX], na = s_wnew[ka * SPEC_MAX];
+                const float4 ob = s_wold[kb * SPEC_MAX], nb = s_wnew[kb * SPEC_MAX];
+                const int ba = s_draw[off + ka].m0, bb = s_draw[off + kb].m0;
+                float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
+                for (int j = wjs + lane; j < wje; j += 64) {
+                    const int jc = j - clo;
+                    const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                    if (j != ba) {
+                        const float eo = pair_crowder(dist2(xj, yj, zj, oa, S), ff);
+                        const float en = pair_crowder(dist2(xj, yj, zj, na, S), ff);
+                        kahan_add(acc_a, comp_a, en - eo);
                     }
-                    const double va = wave_sum_dpp((double)acc_a - (double)comp_a);
-                    const double vb = wave_sum_dpp((double)acc_b - (double)comp_b);
-                    if (lane == 0) {
-                        if (coop > 1) {
-                            __hip_atomic_store(bank + ka * REMC_MAX_COOP + c, slot_pack(va, wtag), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                            __hip_atomic_store(bank + kb * REMC_MAX_COOP + c, slot_pack(vb, wtag), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                        } else {
-                            s_wbase[ka] = va;
-                            s_wbase[kb] = vb;
-                        }
+                    if (j != bb) {
+                        const float eo = pair_crowder(dist2(xj, yj, zj, ob, S), ff);
+                        const float en = pair_crowder(dist2(xj, yj, zj, nb, S), ff);
+                        kahan_add(acc_b, comp_b, en - eo);
                     }
-                    continue;
                 }
-                for (int k = ka; k < w && k <= kb; k += NW) {
-                    const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
-                    const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
-                    float acc = 0.0f, comp = 0.0f;
-                    for (int i = 0; i < nmk; ++i) {
-                        const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
-                        const float *erow = s_etab + (s_wtc[k * SPEC_MAX + i] & 0xff) * REMC_NTYPES;
-                        for (int j = wjs + lane; j < wje; j += 64) {
-                            if (j >= m0k && j < m1k) continue;
-                            const int jc = j - clo; // the partner range lies in the cache
-                            const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
-                            const int tcj = s_jtc[jc];
-                            float eo, en;
-                            if (crow_m || (tcj >> 8) != 0) {
-                                eo = pair_crowder(dist2(xj, yj, zj, o, S), ff);
-                                en = pair_crowder(dist2(xj, yj, zj, n, S), ff);
-                            } else {
-                                const float eij = erow[tcj & 0xff];
-                                const float sij = 0.5f * (n.w + s_js[jc]);
-                                const float sij2 = sij * sij;
-                                const float qq = o.w * s_jq[jc];
-                                eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
-                                en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
-                            }
-                            kahan_add(acc, comp, en - eo);
+                const double va = wave_sum_dpp((double)acc_a - (double)comp_a);
+                const double vb = wave_sum_dpp((double)acc_b - (double)comp_b);
+                if (lane == 0) {
+                    if (coop > 1) {
+                        __hip_atomic_store(bank + ka * REMC_MAX_COOP + c, slot_pack(va, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                        __hip_atomic_store(bank + kb * REMC_MAX_COOP + c, slot_pack(vb, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    } else {
+                        s_wbase[ka] = va;
+                        s_wbase[kb] = vb;
+                    }
+                }
+                k0 = w; // both done
+            }
+            for (int k = k0; k < w; k += NW) {
+                const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
+                const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
+                float acc = 0.0f, comp = 0.0f;
+                for (int i = 0; i < nmk; ++i) {
+                    const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
+                    const float *erow = s_etab + (s_wtc[k * SPEC_MAX + i] & 0xff) * REMC_NTYPES;
+                    for (int j = wjs + lane; j < wje; j += 64) {
+                        if (j >= m0k && j < m1k) continue;
+                        const int jc = j - clo; // the partner range lies in the cache
+                        const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                        const int tcj = s_jtc[jc];
+                        float eo, en;
+                        if (crow_m || (tcj >> 8) != 0) {
+                            eo = pair_crowder(dist2(xj, yj, zj, o, S), ff);
+                            en = pair_crowder(dist2(xj, yj, zj, n, S), ff);
+                        } else {
+                            const float eij = erow[tcj & 0xff];
+                            const float sij = 0.5f * (n.w + s_js[jc]);
+                            const float sij2 = sij * sij;
+                            const float qq = o.w * s_jq[jc];
+                            eo = pair_lj_dh(dist2(xj, yj, zj, o, S), eij, sij2, qq, ff);
+                            en = pair_lj_dh(dist2(xj, yj, zj, n, S), eij, sij2, qq, ff);
                         }
+                        kahan_add(acc, comp, en - eo);
                     }
-                    const double v = wave_sum_dpp((double)acc - (double)comp);
-                    if (lane == 0) {
-                        if (coop > 1)
-                            __hip_atomic_store(bank + k * REMC_MAX_COOP + c, slot_pack(v, wtag), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                        else
-                            s_wbase[k] = v;
-                    }
+                }
+                const double v = wave_sum_dpp((double)acc - (double)comp);
+                if (lane == 0) {
+                    if (coop > 1)
+                        __hip_atomic_store(bank + k * REMC_MAX_COOP + c, slot_pack(v, wtag), __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        s_wbase[k] = v;
                 }
             }
             // pair corrections between the moved bodies of every two slots j < k
-            // (WK is a power of two)
-            const int wk_lg = __builtin_ctz((unsigned int)WK);
-            for (int pr = tid; pr < WK * WK; pr += BLOCK) {
-                const int jj = pr >> wk_lg, kk = pr & (WK - 1);
+            if (tid < WK * WK) {
+                const int jj = tid / WK, kk = tid % WK;
                 if (jj < kk && kk < w) {
                     const int bj = s_draw[off + jj].body, bk = s_draw[off + kk].body;
                     float cv = 0.0f;
                     // the same body: slot kk's proposal depends on slot jj's
                     // move, unless kk is chained to jj (the decision handles it)
-                    bool touches = bj == bk && s_wchain[kk] == 0ull;
+                    bool touches = bj == bk && s_wchain[kk] == 0u;
                     if (bj != bk) {
                         const int nmj = s_draw[off + jj].nm, nmk = s_draw[off + kk].nm;
                         const bool crow = ((s_wtc[jj * SPEC_MAX] | s_wtc[kk * SPEC_MAX]) >> 8) != 0;
@@ -636,7 +625,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         cv = acc - comp;
                         touches = cv != 0.0f;
                     }
-                    if (touches) atomicOr(&s_wint[kk], 1ull << jj);
+                    if (touches) atomicOr(&s_wint[kk], 1u << jj);
                 }
             }
             __syncthreads();
@@ -645,8 +634,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (wave == 0) {
                 // the slot's uniform and masks, read before the wait for the partials
                 const double nll = lane < w ? s_draw[off + lane].nlu : 0.0;
-                const unsigned long long il = lane < w ? s_wint[lane] : 0ull;
-                const unsigned long long chl = lane < w ? s_wchain[lane] : 0ull;
+                const unsigned int il = lane < w ? s_wint[lane] : 0u;
+                const unsigned int chl = lane < w ? s_wchain[lane] : 0u;
                 double dEl = 0.0;
                 if (coop > 1) {
                     // lane k gathers slot k's coop partials (bank[k][m]) and adds
@@ -704,7 +693,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // next window, staged from the updated conformation.
                 const int dl = lane < w ? remc_metropolis_l(dEl, beta, nll) : 0;
                 const unsigned long long acc0 = __ballot(dl != 0);
-                const bool inter = (il & acc0) != 0ull || (chl & ~acc0) != 0ull;
+                const bool inter = (il & (unsigned int)acc0) != 0u || (chl & ~(unsigned int)acc0) != 0u;
                 const unsigned long long stop = __ballot(inter) | (w < 64 ? (1ull << w) : 0ull);
                 const int weff = (int)__builtin_ctzll(stop);
                 if (lane < weff) {
@@ -729,7 +718,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
                 // an accepted chained slot supersedes its (accepted) predecessor's
                 // proposal in what the two moves both change
-                if (lane < weff && dl != 0 && chl != 0ull) s_wnext[__builtin_ctzll(chl)] = lane;
+                if (lane < weff && dl != 0 && chl != 0u) s_wnext[__builtin_ctz(chl)] = lane;
                 if (lane == 0) s_weff = weff;
             }
             __syncthreads();
@@ -1878,9 +1867,6 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
     // the speculative kernel reads its partner range from the cache (the whole replica, or with
     // every member on its own slice the member's slice)
     if (p.spec && p.lds_cl <= 0) return hipErrorInvalidValue;
-    // window slots: a power of two, a multiple of the waves, their staging within one thread each
-    if (p.spec && (p.wk < block / 64 || p.wk > MC_WINDOW_MAX || (p.wk & (p.wk - 1)) != 0 || p.wk * SPEC_MAX > block))
-        return hipErrorInvalidValue;
     if (coop > 1) {
         const int n = n_rep * REMC_SLOTS_PER_REP;
         hipLaunchKernelGGL(k_reset_slots, dim3((n + 255) / 256), dim3(256), 0, s, p.slots, n);
