@@ -124,7 +124,7 @@ void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, do
 constexpr int FULL_TILE = 256;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
-constexpr int REMC_SLOT_K = 32;  // partial sums per member and exchange (window slots; 1 for a split step)
+constexpr int REMC_SLOT_K = 64;  // partial sums per member and exchange (MC window slots, force-biased window values)
 // two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica
 constexpr int REMC_SLOTS_PER_REP = 2 * REMC_SLOT_K * REMC_MAX_COOP;
 

@@ -1124,8 +1124,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
     __shared__ float s_ncen[3]; // the centre of the step's proposal
     __shared__ remc_fb_draws s_fbd[64];
     __shared__ int s_fm0[64], s_fnm[64], s_fcrow[64]; // the drawn body's first bead, bead count, crowder flag
-    // windows of one-bead crowder moves (one slot per wave, at most FBW)
-    constexpr int FBW = NW < 8 ? NW : 8;
+    // windows of one-bead crowder moves (slots k and k + NW on wave k, at most
+    // FBW: four exchanged values per slot within REMC_SLOT_K)
+    constexpr int FBW = 2 * NW < 16 ? 2 * NW : 16;
+    static_assert(4 * FBW <= REMC_SLOT_K, "window exchange rows");
     __shared__ unsigned long long s_felig;               // steps of the batch a window may take
     __shared__ double s_wA[3 * FBW], s_wB[4 * FBW];      // force at the old position; dE and force at the new
     __shared__ float4 s_wold[FBW], s_wnew[FBW], s_wnrel[FBW], s_wncen[FBW];
@@ -1308,7 +1310,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
             w = min(run == 0ull ? 64 : (int)__builtin_ctzll(run), FBW);
         }
         if (CACHE && w >= 2) {
-            // ------------- window of w one-bead crowder moves, slot k = wave k -------------
+            // ----- window of w one-bead crowder moves, slots k and k + NW on wave k -----
             // Each slot's force, dE and new force are taken against the
             // conformation at the window's start; a crowder's pair terms vanish
             // beyond the cutoff, so they are exact unless an accepted earlier
@@ -1318,28 +1320,55 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
             const int js = coop > 1 ? cjs : 0, je = coop > 1 ? cje : N;
             const int bw = it & 63;
             if (wave < w) {
-                const int k = wave, b = s_fm0[bw + k];
-                const float ox = s_jx[b], oy = s_jy[b], oz = s_jz[b];
-                double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+                // slots ka and kb (when it exists) in one pass over the partners
+                const int ka = wave, kb = wave + NW;
+                const bool two = kb < w;
+                const int ba = s_fm0[bw + ka], bb = two ? s_fm0[bw + kb] : ba;
+                const float ax = s_jx[ba], ay = s_jy[ba], az = s_jz[ba];
+                const float bx = s_jx[bb], by = s_jy[bb], bz = s_jz[bb];
+                double fa0 = 0.0, fa1 = 0.0, fa2 = 0.0, fb0 = 0.0, fb1 = 0.0, fb2 = 0.0;
                 for (int j = js + lane; j < je; j += 64) {
-                    if (j == b) continue;
-                    const float dx = min_image(ox - s_jx[j], S.box, S.inv_box);
-                    const float dy = min_image(oy - s_jy[j], S.box, S.inv_box);
-                    const float dz = min_image(oz - s_jz[j], S.box, S.inv_box);
-                    float e, g2;
-                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff, e,
-                            g2);
-                    f0 += -(double)(g2 * dx);
-                    f1 += -(double)(g2 * dy);
-                    f2 += -(double)(g2 * dz);
+                    const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
+                    if (j != ba) {
+                        const float dx = min_image(ax - xj, S.box, S.inv_box);
+                        const float dy = min_image(ay - yj, S.box, S.inv_box);
+                        const float dz = min_image(az - zj, S.box, S.inv_box);
+                        float e, g2;
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                ff, e, g2);
+                        fa0 += -(double)(g2 * dx);
+                        fa1 += -(double)(g2 * dy);
+                        fa2 += -(double)(g2 * dz);
+                    }
+                    if (two && j != bb) {
+                        const float dx = min_image(bx - xj, S.box, S.inv_box);
+                        const float dy = min_image(by - yj, S.box, S.inv_box);
+                        const float dz = min_image(bz - zj, S.box, S.inv_box);
+                        float e, g2;
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                ff, e, g2);
+                        fb0 += -(double)(g2 * dx);
+                        fb1 += -(double)(g2 * dy);
+                        fb2 += -(double)(g2 * dz);
+                    }
                 }
-                f0 = wave_sum_dpp(f0);
-                f1 = wave_sum_dpp(f1);
-                f2 = wave_sum_dpp(f2);
+                fa0 = wave_sum_dpp(fa0);
+                fa1 = wave_sum_dpp(fa1);
+                fa2 = wave_sum_dpp(fa2);
                 if (lane == 0) {
-                    s_wA[3 * k] = f0;
-                    s_wA[3 * k + 1] = f1;
-                    s_wA[3 * k + 2] = f2;
+                    s_wA[3 * ka] = fa0;
+                    s_wA[3 * ka + 1] = fa1;
+                    s_wA[3 * ka + 2] = fa2;
+                }
+                if (two) {
+                    fb0 = wave_sum_dpp(fb0);
+                    fb1 = wave_sum_dpp(fb1);
+                    fb2 = wave_sum_dpp(fb2);
+                    if (lane == 0) {
+                        s_wA[3 * kb] = fb0;
+                        s_wA[3 * kb + 1] = fb1;
+                        s_wA[3 * kb + 2] = fb2;
+                    }
                 }
             }
             __syncthreads();
@@ -1348,8 +1377,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 if (s_abort) break;
             }
             // proposals (every lane of slot k's wave forms the same values)
-            if (wave < w) {
-                const int k = wave, b = s_fm0[bw + k];
+            for (int k = wave; k < w; k += NW) {
+                const int b = s_fm0[bw + k];
                 const remc_fb_draws dk = s_fbd[bw + k];
                 const float rx = s_rx[b], ry = s_ry[b], rz = s_rz[b];
                 const float cx = s_bcx[dk.body], cy = s_bcy[dk.body], cz = s_bcz[dk.body];
@@ -1393,8 +1422,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 }
             }
             __syncthreads();
-            if (wave < w) {
-                const int k = wave, b = s_fm0[bw + k];
+            for (int k = wave; k < w; k += NW) {
                 // which earlier slots reach slot k (lane j looks at slot j)
                 bool touch = false;
                 if (lane < k) {
@@ -1409,40 +1437,77 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 }
                 const unsigned long long tm = __ballot(touch);
                 if (lane == 0) s_wint[k] = (unsigned int)tm;
-                // dE and the force at the new position
-                const float4 o = s_wold[k], n = s_wnew[k];
-                float acc = 0.0f, comp = 0.0f;
-                double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+            }
+            if (wave < w) {
+                // dE and the force at the new position, slots ka and kb in one pass
+                const int ka = wave, kb = wave + NW;
+                const bool two = kb < w;
+                const int ba = s_fm0[bw + ka], bb = two ? s_fm0[bw + kb] : ba;
+                const float4 oa = s_wold[ka], na = s_wnew[ka], ob = s_wold[two ? kb : ka], nb = s_wnew[two ? kb : ka];
+                float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
+                double fa0 = 0.0, fa1 = 0.0, fa2 = 0.0, fb0 = 0.0, fb1 = 0.0, fb2 = 0.0;
                 for (int j = js + lane; j < je; j += 64) {
-                    if (j == b) continue;
                     const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
-                    float eo, en, go, g2;
-                    {
-                        const float dx = min_image(o.x - xj, S.box, S.inv_box);
-                        const float dy = min_image(o.y - yj, S.box, S.inv_box);
-                        const float dz = min_image(o.z - zj, S.box, S.inv_box);
-                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff,
-                                eo, go);
+                    if (j != ba) {
+                        float eo, en, go, g2;
+                        {
+                            const float dx = min_image(oa.x - xj, S.box, S.inv_box);
+                            const float dy = min_image(oa.y - yj, S.box, S.inv_box);
+                            const float dz = min_image(oa.z - zj, S.box, S.inv_box);
+                            pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                    ff, eo, go);
+                        }
+                        const float dx = min_image(na.x - xj, S.box, S.inv_box);
+                        const float dy = min_image(na.y - yj, S.box, S.inv_box);
+                        const float dz = min_image(na.z - zj, S.box, S.inv_box);
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                ff, en, g2);
+                        kahan_add(acc_a, comp_a, en - eo);
+                        fa0 += -(double)(g2 * dx);
+                        fa1 += -(double)(g2 * dy);
+                        fa2 += -(double)(g2 * dz);
                     }
-                    const float dx = min_image(n.x - xj, S.box, S.inv_box);
-                    const float dy = min_image(n.y - yj, S.box, S.inv_box);
-                    const float dz = min_image(n.z - zj, S.box, S.inv_box);
-                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi, ff, en,
-                            g2);
-                    kahan_add(acc, comp, en - eo);
-                    f0 += -(double)(g2 * dx);
-                    f1 += -(double)(g2 * dy);
-                    f2 += -(double)(g2 * dz);
+                    if (two && j != bb) {
+                        float eo, en, go, g2;
+                        {
+                            const float dx = min_image(ob.x - xj, S.box, S.inv_box);
+                            const float dy = min_image(ob.y - yj, S.box, S.inv_box);
+                            const float dz = min_image(ob.z - zj, S.box, S.inv_box);
+                            pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                    ff, eo, go);
+                        }
+                        const float dx = min_image(nb.x - xj, S.box, S.inv_box);
+                        const float dy = min_image(nb.y - yj, S.box, S.inv_box);
+                        const float dz = min_image(nb.z - zj, S.box, S.inv_box);
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), true, 0.0f, 0.0f, 0.0f, inv_xi,
+                                ff, en, g2);
+                        kahan_add(acc_b, comp_b, en - eo);
+                        fb0 += -(double)(g2 * dx);
+                        fb1 += -(double)(g2 * dy);
+                        fb2 += -(double)(g2 * dz);
+                    }
                 }
-                const double dE0 = wave_sum_dpp((double)acc - (double)comp);
-                f0 = wave_sum_dpp(f0);
-                f1 = wave_sum_dpp(f1);
-                f2 = wave_sum_dpp(f2);
+                const double dEa = wave_sum_dpp((double)acc_a - (double)comp_a);
+                fa0 = wave_sum_dpp(fa0);
+                fa1 = wave_sum_dpp(fa1);
+                fa2 = wave_sum_dpp(fa2);
                 if (lane == 0) {
-                    s_wB[4 * k] = dE0;
-                    s_wB[4 * k + 1] = f0;
-                    s_wB[4 * k + 2] = f1;
-                    s_wB[4 * k + 3] = f2;
+                    s_wB[4 * ka] = dEa;
+                    s_wB[4 * ka + 1] = fa0;
+                    s_wB[4 * ka + 2] = fa1;
+                    s_wB[4 * ka + 3] = fa2;
+                }
+                if (two) {
+                    const double dEb = wave_sum_dpp((double)acc_b - (double)comp_b);
+                    fb0 = wave_sum_dpp(fb0);
+                    fb1 = wave_sum_dpp(fb1);
+                    fb2 = wave_sum_dpp(fb2);
+                    if (lane == 0) {
+                        s_wB[4 * kb] = dEb;
+                        s_wB[4 * kb + 1] = fb0;
+                        s_wB[4 * kb + 2] = fb1;
+                        s_wB[4 * kb + 3] = fb2;
+                    }
                 }
             }
             __syncthreads();
