@@ -7,7 +7,9 @@
 //                      Replaces Replica::MCSearch's per-step
 //                      MoleculeDataToDevice/EonDevice round trip
 //                      (Replica.cpp:368-478).
-//   k_mc_fb<BLOCK>     force-biased (smart) MC steps, one workgroup per replica.
+//   k_mc_fb<BLOCK,CACHE>  force-biased (smart) MC steps: 1-8 workgroups per
+//                      replica (large moves split), one-bead crowder moves in
+//                      windows.
 //   k_adjoint_*        gradient of one replica's energy: per-body force and
 //                      torque, d/d(lambda, e0, xi).
 //   k_energy_full      batched full energy, every replica in one launch,
@@ -1072,18 +1074,22 @@ static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, 
 // --------------------------------------------------------------------------
 // force-biased (smart) MC, Rossky, Doll & Friedman (1978), for rigid bodies
 //
-// One workgroup per replica.  The replica's partner beads are cached in LDS
-// when they fit (positions kept current on commit), and wave 0 forms the
-// draws of 64 steps at a time (one step per lane).  Each step: draws;
-// force and torque on the chosen body at its old position (pass A); the
-// proposal d = beta A F + sqrt(2A) n for a translation, the rotation vector
-// theta = beta A tau + sqrt(2A) n for a rotation (thread 0); staging of the
-// new position; dE and the force or torque at the new position (pass B); the
+// coop workgroups (members) per replica, each with its own copy of the
+// replica's state.  CACHE: the replica's beads, relative positions, centres
+// and rotations in LDS (kept current on commit).  Wave 0 forms the draws of
+// 64 steps at a time (one step per lane).  Each step: draws; force and torque
+// on the chosen body at its old position (pass A; a large move split over the
+// members' partner slices, the partials exchanged); the proposal
+// d = beta A F + sqrt(2A) n for a translation, the rotation vector
+// theta = beta A tau + sqrt(2A) n for a rotation (formed by every thread);
+// staging of the new position; dE and the force or torque at the new position
+// (pass B, exchanged likewise); the
 // acceptance exponent x = beta dE + (|-d - beta A F'|^2 - |d - beta A F|^2)/(4A)
 // (accept if x < -ln u); commit.  A = t^2/6 for translations and r^2/6 for
 // rotations, from the replica's translate and rotate steps, so the random
 // part of a move has the plain move's RMS size.  Pair energies in FP32 with
-// Kahan sums of (new - old), forces and torques accumulated in FP64.
+// Kahan sums of (new - old), forces and torques accumulated in FP64.  Runs of
+// one-bead crowder moves are taken in windows (see the window code below).
 // --------------------------------------------------------------------------
 
 // Energy of one pair (pair_lj_dh / pair_crowder, the same operations) and
