@@ -126,6 +126,52 @@ def test_fb_parity_windows_off(oracle, cid):
     assert run_fb_parity(oracle, cid, 400, [0, 7, 19], coop=4, window=False) <= 1
 
 
+def test_fb_uneven_systems_in_one_launch(oracle):
+    """Three systems of different sizes in one force-biased launch with four
+    workgroups per replica: a 100-residue chain among crowders (its moves split
+    over the members), two short chains among crowders, and crowders only (every
+    move in windows).  Every replica follows the oracle."""
+    rng = np.random.default_rng(41)
+    ff = ffm.ForceField.synthetic(41)
+    systems = [
+        synth.build_system([synth.chain_body(100, rng)] + [synth.crowder_body() for _ in range(60)], 120.0, rng),
+        synth.build_system([synth.chain_body(30, rng), synth.chain_body(20, rng)] +
+                           [synth.crowder_body() for _ in range(40)], 90.0, rng),
+        synth.build_system([synth.crowder_body() for _ in range(50)], 70.0, rng),
+    ]
+    ladders = [synth.geometric_ladder(300.0, 450.0, 4) for _ in systems]
+    wl = synth.Workload(name="uneven", systems=systems, temperatures=ladders, translate_step=2.0, rotate_step=0.2,
+                        ff=ff)
+    nsteps = 300
+    a_t, a_r = mobilities(wl)
+    e = engine_for(wl, seed=SEED)
+    e.set_mc_coop(4)
+    e.set_recording(nsteps)
+    e.mc_steps_fb(nsteps)
+    e.synchronize()
+    energies = e.energies()
+    diverged = 0
+    r = 0
+    for s, (sa, lad) in enumerate(zip(systems, ladders)):
+        o = oracle.OracleSystem(sa, ff)
+        for k, T in enumerate(lad):
+            say("fb uneven system", s, "replica", r)
+            orep = oracle.OracleReplica(o, T, wl.translate_step, wl.rotate_step, r)
+            orec = orep.mc_fb(SEED, nsteps, a_t, a_r)
+            n_eq, div, kinds = compare_fb(e.records(r), orec, T)
+            st = e.state(r)
+            if not div:
+                assert np.max(np.abs(st["pos"] - orep.pos)) < 2e-3, r
+                assert abs(energies[r] - orep.energy) < 1e-3 * kT(T) + 1e-5 * abs(orep.energy)
+            else:
+                diverged += 1
+            want = o.energy(st["pos"])
+            assert abs(energies[r] - want) <= 1e-5 * abs(want) + 1e-4 * kT(T)
+            r += 1
+    e.close()
+    assert diverged <= 1
+
+
 def test_fb_rejects_bad_block():
     wl = synth.config(1)
     e = engine_for(wl)
