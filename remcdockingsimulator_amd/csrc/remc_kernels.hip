@@ -372,55 +372,65 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             refill = ((~(elig >> off)) & rest) == 0ull;
         }
         if (refill) {
-            // wave 0 forms the draws of the next DRAW_BATCH steps, one per lane
-            if (wave == 0) {
-                bool elig = false;
-                int body_l = -1;
-                if (it + lane < nsteps) {
-                    const remc_draws dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
-                    StepDraw &sd = s_draw[lane];
-                    sd.nlu = remc_neglog_u(dd.u);
-                    sd.body = body_l = dd.body;
-                    sd.rotate = dd.rotate;
-                    if (dd.rotate) {
+            // the draws of the next DRAW_BATCH steps, one step per lane, formed
+            // by four waves side by side: each forms the counter words itself
+            // and then its part (wave 0 the body, its bead range, the window
+            // eligibility and the same-body links; wave 1 a rotation's
+            // quaternion and coefficients; wave 2 a translation; wave 3 -ln u)
+            if (wave < 4) {
+                const bool valid = it + lane < nsteps;
+                remc_draws dd{};
+                if (valid) dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+                StepDraw &sd = s_draw[lane];
+                if (wave == 0) {
+                    bool el = false;
+                    const int body_l = valid ? dd.body : -1;
+                    if (valid) {
+                        sd.body = dd.body;
+                        sd.rotate = dd.rotate;
+                        const int gb = S.body_off + dd.body;
+                        sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
+                        sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
+                        if (SPEC) el = sd.nm <= SPEC_MAX;
+                    }
+                    if (SPEC) {
+                        const unsigned long long m = __ballot(el);
+                        if (lane == 0) s_elig = m;
+                        // the latest two earlier steps of the batch that move the same
+                        // body: the lanes whose body index agrees with this lane's in
+                        // every bit (one ballot per bit); lanes past the launch's last
+                        // step lie above every valid lane, so never count as earlier
+                        const int nbits = S.n_bodies > 1 ? 32 - __builtin_clz((unsigned int)(S.n_bodies - 1)) : 0;
+                        unsigned long long same = ~0ull;
+                        for (int bt = 0; bt < nbits; ++bt) {
+                            const bool on = ((body_l >> bt) & 1) != 0;
+                            const unsigned long long mb = __ballot(on);
+                            same &= on ? mb : ~mb;
+                        }
+                        const unsigned long long below = same & ((1ull << lane) - 1ull);
+                        int p1 = -1, p2 = -1;
+                        if (below != 0ull) {
+                            p1 = 63 - __builtin_clzll(below);
+                            const unsigned long long rest = below & ~(1ull << p1);
+                            if (rest != 0ull) p2 = 63 - __builtin_clzll(rest);
+                        }
+                        if (valid) {
+                            sd.prev1 = p1;
+                            sd.prev2 = p2;
+                        }
+                    }
+                } else if (wave == 1) {
+                    if (valid && dd.rotate) {
                         double qq[4], cf[9];
                         remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
                         remc_qcoeffs(qq, cf);
                         for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
                         for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
-                    } else {
-                        remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
                     }
-                    const int gb = S.body_off + dd.body;
-                    sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
-                    sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
-                    if (SPEC) elig = sd.nm <= SPEC_MAX;
-                }
-                if (SPEC) {
-                    const unsigned long long m = __ballot(elig);
-                    if (lane == 0) s_elig = m;
-                    // the latest two earlier steps of the batch that move the same
-                    // body: the lanes whose body index agrees with this lane's in
-                    // every bit (one ballot per bit); lanes past the launch's last
-                    // step lie above every valid lane, so never count as earlier
-                    const int nbits = S.n_bodies > 1 ? 32 - __builtin_clz((unsigned int)(S.n_bodies - 1)) : 0;
-                    unsigned long long same = ~0ull;
-                    for (int bt = 0; bt < nbits; ++bt) {
-                        const bool on = ((body_l >> bt) & 1) != 0;
-                        const unsigned long long mb = __ballot(on);
-                        same &= on ? mb : ~mb;
-                    }
-                    const unsigned long long below = same & ((1ull << lane) - 1ull);
-                    int p1 = -1, p2 = -1;
-                    if (below != 0ull) {
-                        p1 = 63 - __builtin_clzll(below);
-                        const unsigned long long rest = below & ~(1ull << p1);
-                        if (rest != 0ull) p2 = 63 - __builtin_clzll(rest);
-                    }
-                    if (it + lane < nsteps) {
-                        s_draw[lane].prev1 = p1;
-                        s_draw[lane].prev2 = p2;
-                    }
+                } else if (wave == 2) {
+                    if (valid && !dd.rotate) remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
+                } else {
+                    if (valid) sd.nlu = remc_neglog_u(dd.u);
                 }
             }
             b0 = it;
