@@ -1295,6 +1295,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         ++n_x;
     };
 
+    int fb0 = 0, fhave = 0; // s_fbd holds the draws of steps [fb0, fb0 + fhave) of this launch
     for (int it = 0; it < nsteps; ++it, ++step) {
         // the thread index re-read opaquely in every step, as in k_mc: the
         // addresses derived from it are formed where used, not held in registers
@@ -1302,7 +1303,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         asm volatile("" : "+v"(tid));
         const int lane = tid & 63, wave = tid >> 6;
         const unsigned long long t_step = p.rec ? wall_clock64() : 0ull;
-        if ((it & 63) == 0) {
+        int off = it - fb0;
+        bool refill = off >= fhave;
+        if (CACHE && p.spec && !refill && off + FBW > fhave && fb0 + fhave < nsteps) {
+            // fewer than FBW draws left: form the next batch now if the run of
+            // window-eligible steps from here reaches the batch's end
+            const unsigned long long rest = (1ull << (fhave - off)) - 1ull; // fhave - off < FBW <= 16
+            refill = ((~(s_felig >> off)) & rest) == 0ull;
+        }
+        if (refill) {
             // wave 0 forms the draws of the next 64 steps, one per lane, with the drawn body's bead range
             if (wave == 0 && it + lane < nsteps) {
                 const remc_fb_draws dd = remc_fb_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
@@ -1317,12 +1326,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 const unsigned long long m = __ballot(el);
                 if (lane == 0) s_felig = m;
             }
+            fb0 = it;
+            fhave = min(64, nsteps - it);
+            off = 0;
             __syncthreads();
         }
         // a window of w >= 2 one-bead crowder moves, within the batch
         int w = 0;
         if (CACHE && p.spec) {
-            const unsigned long long run = ~(s_felig >> (it & 63));
+            const unsigned long long run = ~(s_felig >> off);
             w = min(run == 0ull ? 64 : (int)__builtin_ctzll(run), FBW);
         }
         if (CACHE && w >= 2) {
@@ -1334,7 +1346,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
             // or new position, either end) or moved the same body: the window
             // ends before the first such slot, which starts the next one.
             const int js = coop > 1 ? cjs : 0, je = coop > 1 ? cje : N;
-            const int bw = it & 63;
+            const int bw = off;
             if (wave < w) {
                 // slots ka and kb (when it exists) in one pass over the partners
                 const int ka = wave, kb = wave + NW;
@@ -1634,10 +1646,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
             step += weff - 1;
             continue;
         }
-        const remc_fb_draws d = s_fbd[it & 63];
+        const remc_fb_draws d = s_fbd[off];
         const int body = d.body;
-        const int m0 = s_fm0[it & 63], nm = s_fnm[it & 63], m1 = m0 + nm;
-        const bool crow_m = s_fcrow[it & 63] != 0;
+        const int m0 = s_fm0[off], nm = s_fnm[off], m1 = m0 + nm;
+        const bool crow_m = s_fcrow[off] != 0;
         // the body's centre, needed by the proposal after pass A
         const float cx0 = cache_bodies ? s_bcx[body] : gcx[body], cy0 = cache_bodies ? s_bcy[body] : gcy[body],
                     cz0 = cache_bodies ? s_bcz[body] : gcz[body];
