@@ -45,7 +45,6 @@ __device__ __forceinline__ float min_image(float d, float box, float inv_box)
 // 4 is exact, the result is the same bits as forming 4 x6 (x6 - 1) and E).
 __device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, float qq, const DevFF &ff)
 {
-    r2 = fmaxf(r2, 1e-36f);
     float rinv = __builtin_amdgcn_rsqf(r2);
     float r = r2 * rinv;
     float s2 = sij2 * (rinv * rinv);
@@ -54,7 +53,9 @@ __device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, fl
     bool flip = (eij4 > 0.0f) & (x6 > 0.5f);            // r < 2^(1/6) sigma
     float lj = eij4 * (flip ? base + 0.5f : -base);     // E (LJ + 2) when flipped, else -E LJ
     float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
-    // coincident beads overflow to inf (or 0*inf); cap so compensated sums stay finite
+    // coincident beads (r2 = 0: rinv = inf, r = 0 * inf) give NaN; the cap
+    // (IEEE minNum returns the number) turns it, like any overflow, into 1e30
+    // so compensated sums stay finite
     return fminf(lj + dh, 1e30f);
 }
 
