@@ -866,13 +866,18 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         // Moves with few pair terms are evaluated whole by every member (no
         // exchange); larger ones are split over the members' partner slices.
         const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
-        const int js = split ? cjs : 0, je = split ? cje : N;
-        // this member's share of dE = sum_{i in body} sum_{j in slice, not in body} e(new_i,j) - e(old_i,j),
-        // split into groups of moved beads x G lanes over the slice's partner
-        // beads (the slice without the moved body's overlap [ovs, ovs + ov))
-        const int ovs = max(js, m0);
-        const int ov = max(0, min(je, m1) - ovs);
-        const int nj = (je - js) - ov;
+        // this member's share of dE = sum_{i in body} sum_{j in share} e(new_i,j) - e(old_i,j),
+        // split into groups of moved beads x G lanes over the share's partner
+        // beads: an even part of the N - nm partners (index v, bead v, or
+        // v + nm past the moved body) when the member can read every bead,
+        // else its cached slice without the moved body's overlap [ovs, ovs + ov)
+        const bool even = split && !(CACHE && p.cache_slice);
+        const int np = N - nm;
+        const int js = !split ? 0 : even ? (int)(((long long)c * np) / coop) : cjs;
+        const int je = !split ? N : even ? (int)(((long long)(c + 1) * np) / coop) : cje;
+        const int ovs = even ? m0 : max(js, m0);
+        const int ov = even ? nm : max(0, min(je, m1) - ovs);
+        const int nj = even ? je - js : (je - js) - ov;
         int G, kper;
         pick_split<BLOCK>(nm, nj, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
@@ -1664,13 +1669,17 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         }
         __syncthreads();
 
-        // A large move is split over the members' partner slices [cjs, cje),
-        // the partial forces, torques and dE then exchanged; a small one is
+        // A large move is split over the members' shares of the partners, the
+        // partial forces, torques and dE then exchanged; a small one is
         // evaluated whole by every member, with the same result in each.
         const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
-        const int js = split ? cjs : 0, je = split ? cje : N;
-        const int ovs = max(js, m0), ov = max(0, min(je, m1) - ovs);
-        const int nj = (je - js) - ov;
+        // a member's share: an even part of the N - nm partners (index v, bead
+        // v, or v + nm past the moved body)
+        const int np = N - nm;
+        const int js = split ? (int)(((long long)c * np) / coop) : 0;
+        const int je = split ? (int)(((long long)(c + 1) * np) / coop) : np;
+        const int ovs = m0, ov = nm;
+        const int nj = je - js;
 
         // pass A: force and torque at the old position; the (moved bead x
         // partner bead) pairs split over the workgroup as in the MC kernel
