@@ -2383,24 +2383,38 @@ __global__ void k_adjoint_reduce(int n_beads, int n_chunks, const double *part_g
     bead_grad[t] = g;
 }
 
-__global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad, const double *param_part, int n_parts,
-                                 double *force_torque, double *dparams)
+// One wave per body: the lanes stride over its beads, then wave sums (fixed
+// order, so every call gives the same bits); the first wave of workgroup 0 also
+// adds the parameter partials.
+__global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int replica, const double *bead_grad,
+                                                        const double *param_part, int n_parts, double *force_torque,
+                                                        double *dparams)
 {
     const DevRep R = p.rep[replica];
     const DevSys S = p.sys[R.sys];
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m == 0) {
-        for (int k = 0; k < 3; ++k) {
-            double t = 0.0;
-            for (int a = 0; a < n_parts; ++a) t += param_part[3 * a + k];
-            dparams[k] = t;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && wave == 0) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        for (int a = lane; a < n_parts; a += 64) {
+            t0 += param_part[3 * a];
+            t1 += param_part[3 * a + 1];
+            t2 += param_part[3 * a + 2];
+        }
+        t0 = wave_sum_dpp(t0);
+        t1 = wave_sum_dpp(t1);
+        t2 = wave_sum_dpp(t2);
+        if (lane == 0) {
+            dparams[0] = t0;
+            dparams[1] = t1;
+            dparams[2] = t2;
         }
     }
+    const int m = blockIdx.x * (blockDim.x >> 6) + wave;
     if (m >= S.n_bodies) return;
     const int b0 = p.bstart[S.body_off + m], b1 = p.bend[S.body_off + m];
     const float *rx = p.rx + R.bead_off, *ry = p.ry + R.bead_off, *rz = p.rz + R.bead_off;
     double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
-    for (int i = b0; i < b1; ++i) {
+    for (int i = b0 + lane; i < b1; i += 64) {
         const double f0 = -bead_grad[3 * i], f1 = -bead_grad[3 * i + 1], f2 = -bead_grad[3 * i + 2];
         F[0] += f0; F[1] += f1; F[2] += f2;
         const double ax = rx[i], ay = ry[i], az = rz[i];
@@ -2409,9 +2423,14 @@ __global__ void k_adjoint_bodies(KParams p, int replica, const double *bead_grad
         T[2] += ax * f1 - ay * f0;
     }
     for (int k = 0; k < 3; ++k) {
-        force_torque[6 * m + k] = F[k];
-        force_torque[6 * m + 3 + k] = T[k];
+        F[k] = wave_sum_dpp(F[k]);
+        T[k] = wave_sum_dpp(T[k]);
     }
+    if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+            force_torque[6 * m + k] = F[k];
+            force_torque[6 * m + 3 + k] = T[k];
+        }
 }
 
 size_t adjoint_scratch_doubles(int max_beads, int max_bodies)
@@ -2437,7 +2456,7 @@ hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int
                        param_part);
     hipLaunchKernelGGL(k_adjoint_reduce, dim3((3 * n_beads + 255) / 256), dim3(256), 0, s, n_beads, tj, part_grad,
                        bead_grad);
-    hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 63) / 64), dim3(64), 0, s, p, replica, bead_grad, param_part,
+    hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 3) / 4), dim3(256), 0, s, p, replica, bead_grad, param_part,
                        ti * tj, force_torque, dparams);
     return hipGetLastError();
 }
