@@ -2252,9 +2252,9 @@ hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
 // --------------------------------------------------------------------------
 // adjoint: gradient of the full energy of one replica
 //
-// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of tj
-// beads: ADJ_TJ, or N/16 for large systems so that a workgroup has enough
-// work); thread i of a tile accumulates, over the chunk's partner beads
+// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of N/64
+// beads rounded up to 32, between ADJ_TJ and ADJ_TJ_MAX: about 64 chunks, so
+// that the grid holds several waves per CU); thread i of a tile accumulates, over the chunk's partner beads
 // (staged in LDS), the FP32 pair derivatives (same branches as the pair term)
 // into FP64 dE/dx_i, and the parameter derivatives
 // dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij,
@@ -2270,14 +2270,24 @@ struct AdjFF {
 };
 
 constexpr int ADJ_TI = 64;       // beads per tile (one wave)
-constexpr int ADJ_TJ = 128;      // smallest partner chunk
+constexpr int ADJ_TJ = 32;       // smallest partner chunk
 constexpr int ADJ_TJ_MAX = 2048; // largest partner chunk (LDS)
+constexpr int ADJ_CHUNKS = 64;   // partner chunks a system is cut into (fewer when small)
 
 __device__ __host__ inline int adjoint_chunk(int n_beads)
 {
-    int c = (n_beads + 15) / 16;
-    c = ((c + 63) / 64) * 64;
+    int c = (n_beads + ADJ_CHUNKS - 1) / ADJ_CHUNKS;
+    c = ((c + ADJ_TJ - 1) / ADJ_TJ) * ADJ_TJ;
     return c < ADJ_TJ ? ADJ_TJ : (c > ADJ_TJ_MAX ? ADJ_TJ_MAX : c);
+}
+
+// Most partner chunks of any system of up to max_beads beads: ADJ_CHUNKS
+// (a chunk is at least n/ADJ_CHUNKS beads, or ADJ_TJ when n < ADJ_CHUNKS *
+// ADJ_TJ), more only where the chunk reaches ADJ_TJ_MAX.
+inline size_t adjoint_max_chunks(int max_beads)
+{
+    const size_t big = (size_t)(max_beads + ADJ_TJ_MAX - 1) / ADJ_TJ_MAX;
+    return big > (size_t)ADJ_CHUNKS ? big : (size_t)ADJ_CHUNKS;
 }
 
 __global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, int replica, int chunk,
@@ -2435,17 +2445,17 @@ __global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int replica, 
 
 size_t adjoint_scratch_doubles(int max_beads, int max_bodies)
 {
-    const size_t ti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, tj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
+    const size_t ti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, tj = adjoint_max_chunks(max_beads);
     return 3 * tj * (size_t)max_beads + 3 * (size_t)max_beads + 3 * ti * tj + 6 * (size_t)max_bodies + 3;
 }
 
 hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
                           int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s)
 {
-    // chunk >= ADJ_TJ, so the scratch sized for ADJ_TJ chunks of the largest system holds every split
+    // every system of the context has at most adjoint_max_chunks(max_beads) chunks: the scratch's stride
     const int chunk = adjoint_chunk(n_beads);
     const int ti = (n_beads + ADJ_TI - 1) / ADJ_TI, tj = (n_beads + chunk - 1) / chunk;
-    const size_t mtj = (size_t)(max_beads + ADJ_TJ - 1) / ADJ_TJ;
+    const size_t mtj = adjoint_max_chunks(max_beads);
     double *part_grad = scratch, *bead_grad = part_grad + 3 * mtj * (size_t)max_beads;
     double *param_part = bead_grad + 3 * (size_t)max_beads;
     const AdjFF af = *(const AdjFF *)adj_ff;
