@@ -145,6 +145,8 @@ def main():
                     help="force-biased MC steps per replica timed after the main run (0 = skip)")
     ap.add_argument("--adjoint-calls", type=int, default=20,
                     help="adjoint (gradient) calls timed after the main run (0 = skip)")
+    ap.add_argument("--energy-calls", type=int, default=20,
+                    help="full-energy refreshes of every replica timed after the main run (0 = skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
@@ -296,6 +298,25 @@ def main():
                "work_per_call": "2 P_full ordered pair derivatives (P_full = %d)" % wl.pairs_full()}
         log("adjoint: %.3f ms per call (kernels)" % adj["kernel_avg_ms"])
 
+    # ---------------------------------------------------- full energy ----
+    full = None
+    if args.energy_calls > 0:
+        e.refresh_energies()  # warm
+        e.timing_enable(True)
+        e.synchronize()
+        t8 = time.perf_counter()
+        for k in range(args.energy_calls):
+            e.refresh_energies()
+        t9 = time.perf_counter()
+        ems, enl = e.timing(Engine.FULL)
+        terms = sum(len(wl.temperatures[s]) * wl.pairs_full(s) for s in range(len(wl.systems)))
+        full = {"calls": args.energy_calls, "kernel_avg_ms": ems / max(enl, 1),
+                "pair_terms_per_s": terms / (ems / max(enl, 1) / 1e3),
+                "calls_per_s_wall": args.energy_calls / (t9 - t8),
+                "kernel": "k_energy_full + k_sum_partials (full FP32-Kahan energy of every replica, FP64 totals)",
+                "work_per_call": "%d inter-body pair terms over all replicas" % terms}
+        log("full energy: %.3f ms per refresh of all replicas (kernels)" % full["kernel_avg_ms"])
+
     # ------------------------------------------------------------ cpu ----
     cpu = None
     if not args.no_cpu and len(wl.systems) == 1 and gpu_records is not None:
@@ -334,6 +355,7 @@ def main():
         "cpu_baseline": cpu,
         "force_biased": fb,
         "adjoint": adj,
+        "full_energy": full,
         "energy_mean": float(np.mean(energies)),
     }
     e.close()

@@ -379,7 +379,7 @@ int finalize(remc_ctx *c)
     HIP_TRY(dalloc(&c->d_exround, c->systems.size()));
     HIP_TRY(dalloc(&c->d_exoff, c->systems.size()));
     HIP_TRY(dalloc(&c->d_exswaps, c->systems.size()));
-    HIP_TRY(dalloc(&c->d_full_partials, (size_t)R * c->max_tiles));
+    HIP_TRY(dalloc(&c->d_full_partials, (size_t)R * full_units(c->max_tiles) * FULL_SPLIT_MAX));
     HIP_TRY(dalloc(&c->d_out, (size_t)R));
     HIP_TRY(dalloc(&c->d_rec_start, (size_t)R));
 

@@ -122,6 +122,11 @@ size_t adjoint_ff_size();
 void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh);
 
 constexpr int FULL_TILE = 256;
+// full-energy work units per replica of a system with `tiles` bead tiles:
+// row tile x offset 0..tiles/2 of the half-window tile schedule
+__host__ __device__ inline int full_units(int tiles) { return tiles * (tiles / 2 + 1); }
+// largest number of partner sub-ranges a full-energy unit is cut into
+constexpr int FULL_SPLIT_MAX = 4;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
 constexpr int REMC_SLOT_K = 64;  // partial sums per member and exchange (MC window slots, force-biased window values)

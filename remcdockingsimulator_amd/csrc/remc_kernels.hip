@@ -2035,7 +2035,16 @@ int mc_static_lds(int block)
 // --------------------------------------------------------------------------
 // batched full energy
 // --------------------------------------------------------------------------
-__global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int max_tiles, double *partials, int noncrowder_only)
+// One workgroup per (row tile a, offset bo) unit of the half-window tile
+// schedule: tile a against tile (a + bo) mod G for bo = 0..G/2 (the last
+// offset only for a < G/2 when G is even), so every unordered tile pair is
+// visited once.  Unit x = a (G/2 + 1) + bo of each replica writes one FP64
+// partial (0 for the skipped units); k_sum_partials adds them in a fixed
+// order.  When the grid is small (a few replicas of a thousand beads) each
+// unit's partner tile is further cut into `split` equal sub-ranges, one
+// workgroup each, so that every SIMD holds several waves.
+__global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int stride, int split, double *partials,
+                                                           int noncrowder_only)
 {
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
     __shared__ float s_x[FULL_TILE], s_y[FULL_TILE], s_z[FULL_TILE], s_q[FULL_TILE], s_s[FULL_TILE];
@@ -2043,12 +2052,18 @@ __global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int max_ti
     __shared__ double s_red[FULL_TILE / 64];
 
     const int tid = threadIdx.x;
-    const int a = blockIdx.x, r = blockIdx.y;
+    const int x = blockIdx.x / split, sub = blockIdx.x % split, r = blockIdx.y;
     const DevRep R = p.rep[r];
     const DevSys S = p.sys[R.sys];
     const int N = S.n_beads;
     const int G = (N + FULL_TILE - 1) / FULL_TILE;
-    if (a >= G) return; // whole workgroup leaves before any barrier
+    const int h = G / 2;
+    if (x >= full_units(G)) return; // whole workgroup leaves before any barrier
+    const int a = x / (h + 1), bo = x % (h + 1);
+    if ((G % 2 == 0) && bo == h && bo > 0 && a >= h) {
+        if (tid == 0) partials[(size_t)r * stride + blockIdx.x] = 0.0;
+        return;
+    }
     const DevFF ff = p.ff;
     for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += FULL_TILE) s_etab[k] = p.etab[k];
 
@@ -2065,23 +2080,19 @@ __global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int max_ti
     const float *erow = s_etab + (tci & 0xff) * REMC_NTYPES;
 
     float acc = 0.0f, comp = 0.0f;
-    const int h = G / 2;
-    for (int bo = 0; bo <= h; ++bo) {
+    {
         const int b = (a + bo) % G;
-        if ((G % 2 == 0) && bo == h && bo > 0 && a >= h) break;
-        __syncthreads();
-        const int j0 = b * FULL_TILE;
+        const int sw = FULL_TILE / split;
+        const int j0 = b * FULL_TILE + sub * sw;
         const int jt = j0 + tid;
-        if (jt < N) {
+        if (tid < sw && jt < N) {
             s_x[tid] = gx[jt]; s_y[tid] = gy[jt]; s_z[tid] = gz[jt];
             s_q[tid] = qs[jt]; s_s[tid] = sgs[jt]; s_tc[tid] = tcs[jt]; s_body[tid] = bodies[jt];
-        } else {
-            s_body[tid] = -2;
         }
         __syncthreads();
-        if (!vi) continue;
-        const int nj = min(FULL_TILE, N - j0);
-        const int kstart = (bo == 0) ? tid + 1 : 0;
+        // partner sub-range [j0, j0 + nj): only loaded entries are read
+        const int nj = vi ? min(sw, N - j0) : 0;
+        const int kstart = (bo == 0) ? max(tid + 1 - sub * sw, 0) : 0;
         for (int k = kstart; k < nj; ++k) {
             if (s_body[k] == bi) continue;
             const int tcj = s_tc[k];
@@ -2107,30 +2118,38 @@ __global__ __launch_bounds__(FULL_TILE) void k_energy_full(KParams p, int max_ti
     if (tid == 0) {
         double t = 0.0;
         for (int w = 0; w < FULL_TILE / 64; ++w) t += s_red[w];
-        partials[(size_t)r * max_tiles + a] = t;
+        partials[(size_t)r * stride + blockIdx.x] = t;
     }
 }
 
-__global__ void k_sum_partials(KParams p, int n_rep, int max_tiles, const double *partials, double *out, int write_E)
+// one wave per replica: lane l adds units l, l+64, ... in order, then a
+// fixed butterfly across the wave (the same total on every call)
+__global__ __launch_bounds__(64) void k_sum_partials(KParams p, int stride, int split, const double *partials,
+                                                     double *out, int write_E)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rep) return;
+    const int r = blockIdx.x, lane = threadIdx.x;
     const DevSys S = p.sys[p.rep[r].sys];
-    const int G = (S.n_beads + FULL_TILE - 1) / FULL_TILE;
+    const int U = full_units((S.n_beads + FULL_TILE - 1) / FULL_TILE) * split;
     double t = 0.0;
-    for (int a = 0; a < G; ++a) t += partials[(size_t)r * max_tiles + a];
-    if (out) out[r] = t;
-    if (write_E) p.E[r] = t;
+    for (int x = lane; x < U; x += 64) t += partials[(size_t)r * stride + x];
+    t = wave_sum(t);
+    if (lane == 0) {
+        if (out) out[r] = t;
+        if (write_E) p.E[r] = t;
+    }
 }
 
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
                               int noncrowder_only, int write_E, hipStream_t s)
 {
     if (n_rep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_energy_full, dim3(max_tiles, n_rep), dim3(FULL_TILE), 0, s, p, max_tiles, partials,
-                       noncrowder_only);
-    hipLaunchKernelGGL(k_sum_partials, dim3((n_rep + 63) / 64), dim3(64), 0, s, p, n_rep, max_tiles, partials, out,
-                       write_E);
+    const int max_units = full_units(max_tiles);
+    const long long units = (long long)max_units * n_rep;
+    const int split = units >= 2048 ? 1 : units >= 1024 ? 2 : FULL_SPLIT_MAX;
+    const int stride = max_units * FULL_SPLIT_MAX;
+    hipLaunchKernelGGL(k_energy_full, dim3(max_units * split, n_rep), dim3(FULL_TILE), 0, s, p, stride, split,
+                       partials, noncrowder_only);
+    hipLaunchKernelGGL(k_sum_partials, dim3(n_rep), dim3(64), 0, s, p, stride, split, partials, out, write_E);
     return hipGetLastError();
 }
 
