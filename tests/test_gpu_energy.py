@@ -69,6 +69,26 @@ def test_config5_full_energy(oracle):
     e.close()
 
 
+def test_full_energy_partner_cuts(oracle):
+    """The config-4 system (6 bead tiles, 24 work units per replica) with 32,
+    48 and 96 replicas: the launch cuts each unit's partner tile into 4, 2 and
+    1 sub-ranges (fewer than 1024, 2048 and at least 2048 units).  Every
+    replica matches the oracle and every cut gives the same totals."""
+    wl = synth.config(4)
+    sa = wl.systems[0]
+    want = oracle.OracleSystem(sa, wl.ff).energy(sa.pos)
+    totals = []
+    for n_rep in (32, 48, 96):
+        say("config 4 system, replicas", n_rep)
+        e = _engine_single(sa, wl.ff, n_rep)
+        full = e.energy_all()
+        assert np.all(np.abs(full - want) <= REL * abs(want) + ABS), (n_rep, full[:4], want)
+        assert np.all(full == full[0]), n_rep  # identical conformations, fixed summation order
+        totals.append(full[0])
+        e.close()
+    assert max(totals) - min(totals) <= REL * abs(want) + ABS, totals
+
+
 def test_ragged_systems_in_one_launch(oracle):
     """Several systems of different sizes (uneven systems of config 3's kind)
     evaluated in one batched launch."""
