@@ -43,20 +43,34 @@ __device__ __forceinline__ float min_image(float d, float box, float inv_box)
 // Energy of one non-crowder pair from squared distance.  eij4 = 4 E_ij (the
 // table holds the factor 4, so LJ = E (4 x6 (x6 - 1)) costs one FMA: scaling by
 // 4 is exact, the result is the same bits as forming 4 x6 (x6 - 1) and E).
+// SKIP_UNCHARGED: the Debye-Hueckel term is formed under a branch on qq != 0.
+// Four residue types of twenty carry a charge, so where the charge product
+// is the same for a whole wave (one partner bead per wave in the full energy,
+// one moved bead per wave in the MC pair loops) most waves skip the exp2 and
+// its multiplies.  The skipped term is +-0, so the sum is the same number;
+// coincident beads still give NaN from the LJ part (r2 = 0: rinv = inf,
+// x6 = inf, inf*inf - inf) and hence the 1e30 cap.  The plain per-replica
+// kernel forms every term.
+template <bool SKIP_UNCHARGED = true>
 __device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, float qq, const DevFF &ff)
 {
     float rinv = __builtin_amdgcn_rsqf(r2);
-    float r = r2 * rinv;
     float s2 = sij2 * (rinv * rinv);
     float x6 = s2 * s2 * s2;
     float base = __builtin_fmaf(x6, x6, -x6);           // x6 (x6 - 1)
     bool flip = (eij4 > 0.0f) & (x6 > 0.5f);            // r < 2^(1/6) sigma
-    float lj = eij4 * (flip ? base + 0.5f : -base);     // E (LJ + 2) when flipped, else -E LJ
-    float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+    float e = eij4 * (flip ? base + 0.5f : -base);      // E (LJ + 2) when flipped, else -E LJ
+    if (!SKIP_UNCHARGED || qq != 0.0f) {
+        float r = r2 * rinv;
+        // keeps the exp2 inside the branch (otherwise it is hoisted above it
+        // and the branch becomes a select)
+        if (SKIP_UNCHARGED) asm volatile("" : "+v"(r));
+        e = e + qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+    }
     // coincident beads (r2 = 0: rinv = inf, r = 0 * inf) give NaN; the cap
     // (IEEE minNum returns the number) turns it, like any overflow, into 1e30
     // so compensated sums stay finite
-    return fminf(lj + dh, 1e30f);
+    return fminf(e, 1e30f);
 }
 
 // Crowder term C (sigma_c / r)^6 inside the cutoff as crow_k (1/r^2)^3.  r2 = 0
@@ -1120,7 +1134,7 @@ __device__ __forceinline__ void pair_eg(float r2, bool crow, float eij, float si
         g = fmaxf(-6.0f * e * rc, -1e30f); // d(C r^-6)/dr / r = -6 E / r^2
         return;
     }
-    e = pair_lj_dh(r2, eij, sij2, qq, ff);
+    e = pair_lj_dh<false>(r2, eij, sij2, qq, ff);
     r2 = fmaxf(r2, 1e-36f);
     const float rinv = __builtin_amdgcn_rsqf(r2);
     const float r = r2 * rinv;
@@ -2203,7 +2217,7 @@ __global__ __launch_bounds__(B) void k_energy_plain(KParams p, int replica, floa
                 sum += pair_crowder(r2, ff);
             } else {
                 const float sij = 0.5f * (si + s_s[k]);
-                sum += pair_lj_dh(r2, s_etab[(tci & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, qi * s_q[k], ff);
+                sum += pair_lj_dh<false>(r2, s_etab[(tci & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, qi * s_q[k], ff);
             }
         }
         if (bx == by) sum *= 0.5f; // diagonal tile counts both orders (CudaFunctions.cu:651-653)
