@@ -472,25 +472,23 @@ def main(argv=None):
     # -------------------------------------------------------- adjoint ----
     adj = None
     if args.adjoint_calls > 0 and single:
-        for r in range(R):
-            e.adjoint(r)  # warm
+        e.adjoint_all()  # warm
         e.timing_enable(True)
         e.synchronize()
         t6 = time.perf_counter()
         for k in range(args.adjoint_calls):
-            for r in range(R):
-                e.adjoint(r)
+            e.adjoint_all()
         t7 = time.perf_counter()
         ams, anl = e.timing(Engine.ADJOINT)
         dpairs = 2 * wl.pairs_full() * R
-        ak = ams / max(anl, 1) * R
+        ak = ams / max(anl, 1)
         aach = dpairs / (ak / 1e3) * FLOPS_PER_PAIR_DERIV / 1e12
         adj = {"calls": args.adjoint_calls, "kernel_avg_ms": ak,
                "pair_derivatives_per_s": dpairs / (ak / 1e3),
                "roofline": {"bound": "valu", "achieved": aach, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                             "frac": aach / PEAK_FP32_TFLOPS, "flops_per_pair": FLOPS_PER_PAIR_DERIV},
                "calls_per_s_wall": args.adjoint_calls / (t7 - t6),
-               "kernel": "adjoint of every replica, one call per replica (forces, torques, dE/d{lambda,e0,xi})",
+               "kernel": "k_adjoint_rows + k_adjoint_bodies: adjoint of every replica in one launch (forces, torques, dE/d{lambda,e0,xi})",
                "work_per_call": "2 P_full ordered pair derivatives per replica (P_full = %d), %d replicas"
                                 % (wl.pairs_full(), R)}
         log("adjoint: %.3f ms per call of all replicas (kernels)" % ak)

@@ -230,6 +230,11 @@ REMC_API int remc_centre_histogram(remc_ctx *ctx, int sys_id, const int *body_ty
  * gradient of E with respect to (lambda, e0, xi) for one replica.  No
  * counterpart in the reference (its gradient-free MC has none). */
 REMC_API int remc_adjoint(remc_ctx *ctx, int replica, double *force_torque, double *dparams);
+/* The same for every replica in one launch: force_torque holds 6 doubles per
+ * body of each replica's system, replicas in order (6 x the total body count
+ * over replicas); dparams 3 per replica.  Either output may be NULL, not both.
+ * No counterpart in the reference. */
+REMC_API int remc_adjoint_all(remc_ctx *ctx, double *force_torque, double *dparams);
 
 /* -------------------------------------------------------------- timing --- */
 /* Kernel timing with HIP events recorded on the context's stream around every

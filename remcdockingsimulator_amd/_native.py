@@ -85,6 +85,7 @@ SIGNATURES = {
     "remc_centre_histogram": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]),
     "remc_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),
+    "remc_adjoint_all": (ctypes.c_int, [_vp, _dp, _dp]),
     "remc_timing_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "remc_synchronize": (ctypes.c_int, [_vp]),

@@ -113,7 +113,8 @@ struct remc_ctx {
     int rec_max = 0;
     unsigned long long *d_slots = nullptr;
     int *d_err = nullptr;
-    double *d_adj = nullptr; // adjoint scratch: bead gradients, tile partials, body forces, parameter gradient
+    double *d_adj = nullptr; // adjoint scratch: bead gradients, tile partials, body forces, parameter gradients
+    size_t adj_cap = 0;      // doubles held by d_adj
     void *d_ana = nullptr;   // analysis scratch (reference conformation, partials, histogram)
     size_t ana_bytes = 0;
     DevFF devff{};
@@ -591,18 +592,45 @@ int autotune_mc(remc_ctx *c)
         size_t bytes;
         void *save;
     };
-    std::vector<Buf> bufs = {
-        {c->d_px, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_py, sizeof(float) * c->total_rep_beads, nullptr},
-        {c->d_pz, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_rx, sizeof(float) * c->total_rep_beads, nullptr},
-        {c->d_ry, sizeof(float) * c->total_rep_beads, nullptr}, {c->d_rz, sizeof(float) * c->total_rep_beads, nullptr},
-        {c->d_cx, sizeof(float) * c->total_rep_bodies, nullptr}, {c->d_cy, sizeof(float) * c->total_rep_bodies, nullptr},
-        {c->d_cz, sizeof(float) * c->total_rep_bodies, nullptr}, {c->d_quat, sizeof(double) * 4 * c->total_rep_bodies, nullptr},
-        {c->d_E, sizeof(double) * R, nullptr}, {c->d_step, sizeof(unsigned long long) * R, nullptr},
-        {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}};
+    // Owns the saved state and the events: on every exit, the error paths
+    // included, the live state is put back from the saves (a no-op after a
+    // completed trial), then the saves and events are released.
+    struct Trials {
+        remc_ctx *c;
+        std::vector<Buf> bufs;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Trials()
+        {
+            for (auto &b : bufs)
+                if (b.save) (void)hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream);
+            (void)hipStreamSynchronize(c->stream);
+            for (auto &b : bufs)
+                if (b.save) (void)hipFree(b.save);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            c->synced_copies = 1;
+        }
+    };
     int rc = sync_copies(c, 1);
     if (rc != REMC_OK) return rc;
-    for (auto &b : bufs) {
-        HIP_TRY(hipMalloc(&b.save, b.bytes));
+    Trials tr{c,
+              {{c->d_px, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_py, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_pz, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_rx, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_ry, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_rz, sizeof(float) * c->total_rep_beads, nullptr},
+               {c->d_cx, sizeof(float) * c->total_rep_bodies, nullptr},
+               {c->d_cy, sizeof(float) * c->total_rep_bodies, nullptr},
+               {c->d_cz, sizeof(float) * c->total_rep_bodies, nullptr},
+               {c->d_quat, sizeof(double) * 4 * c->total_rep_bodies, nullptr},
+               {c->d_E, sizeof(double) * R, nullptr},
+               {c->d_step, sizeof(unsigned long long) * R, nullptr},
+               {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}}};
+    for (auto &b : tr.bufs) {
+        void *save = nullptr;
+        HIP_TRY(hipMalloc(&save, b.bytes));
+        b.save = save; // set only once allocated: the guard copies back from it
         HIP_TRY(hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     KParams p = c->params();
@@ -610,9 +638,8 @@ int autotune_mc(remc_ctx *c)
     const int blocks[3] = {256, 512, 1024};
     double best = 1e30;
     int best_block = 256, best_coop = 1;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
+    HIP_TRY(hipEventCreate(&tr.e0));
+    HIP_TRY(hipEventCreate(&tr.e1));
     const int trial_steps = 256;
     for (int bi = 0; bi < 3; ++bi) {
         const int blk = blocks[bi];
@@ -629,17 +656,18 @@ int autotune_mc(remc_ctx *c)
                 c->synced_copies = 1;
                 rc = mc_launch(c, p, 4, blk, coop); // warm
                 if (rc != REMC_OK) return rc;
-                HIP_TRY(hipEventRecord(e0, c->stream));
+                HIP_TRY(hipEventRecord(tr.e0, c->stream));
                 rc = mc_launch(c, p, trial_steps, blk, coop);
                 if (rc != REMC_OK) return rc;
-                HIP_TRY(hipEventRecord(e1, c->stream));
-                HIP_TRY(hipEventSynchronize(e1));
+                HIP_TRY(hipEventRecord(tr.e1, c->stream));
+                HIP_TRY(hipEventSynchronize(tr.e1));
                 rc = check_err(c);
                 if (rc != REMC_OK) return rc;
                 float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                HIP_TRY(hipEventElapsedTime(&ms, tr.e0, tr.e1));
                 ms_best = std::min(ms_best, ms);
-                for (auto &b : bufs) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+                for (auto &b : tr.bufs)
+                    HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
             }
             if (ms_best < best) {
                 best = ms_best;
@@ -648,11 +676,6 @@ int autotune_mc(remc_ctx *c)
             }
         }
     }
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    for (auto &b : bufs) (void)hipFree(b.save);
-    c->synced_copies = 1;
     c->mc_block = best_block;
     c->mc_coop = best_coop;
     return REMC_OK;
@@ -666,6 +689,38 @@ int ensure_mc_config(remc_ctx *c)
     if (rc != REMC_OK) return rc;
     if (c->mc_coop > cmax)
         return fail(REMC_ERR_LIMIT, "%d workgroups per replica do not fit co-resident (at most %d)", c->mc_coop, cmax);
+    return REMC_OK;
+}
+
+// Adjoint of replicas [first, first + n): their beads occupy bead_stride
+// entries of the replica arrays from bead_base, their bodies n_bodies_out
+// entries from body_base.  Outputs: 6 doubles per body in replica order, 3
+// parameter derivatives per replica.
+int adjoint_range(remc_ctx *c, int first, int n, int max_beads, int max_bodies, int bead_base, size_t bead_stride,
+                  int body_base, int n_bodies_out, double *force_torque, double *dparams)
+{
+    const AdjPlan a = adjoint_plan(n, max_beads, bead_stride);
+    const size_t need = a.part_doubles + a.param_doubles + 6 * (size_t)n_bodies_out + 3 * (size_t)n;
+    if (need > c->adj_cap) {
+        if (c->d_adj) HIP_TRY(hipFree(c->d_adj));
+        c->d_adj = nullptr;
+        c->adj_cap = 0;
+        HIP_TRY(hipMalloc(&c->d_adj, need * sizeof(double)));
+        c->adj_cap = need;
+    }
+    double *ft = c->d_adj + a.part_doubles + a.param_doubles, *dp = ft + 6 * (size_t)n_bodies_out;
+    std::vector<unsigned char> af(adjoint_ff_size());
+    adjoint_ff_fill(af.data(), c->ff.table, c->ff.lambda, c->ff.e0, c->ff.xi, c->ff.lj_cf * c->ff.kbt_cf,
+                    c->ff.dh_cf * c->ff.kbt_cf);
+    {
+        TimedLaunch t(c, 4);
+        HIP_TRY(launch_adjoint(c->params(), af.data(), a, first, n, max_bodies, bead_base, bead_stride, body_base,
+                               c->d_adj, ft, dp, c->stream));
+    }
+    if (force_torque)
+        HIP_TRY(hipMemcpyAsync(force_torque, ft, sizeof(double) * 6 * n_bodies_out, hipMemcpyDeviceToHost, c->stream));
+    if (dparams) HIP_TRY(hipMemcpyAsync(dparams, dp, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
 }
 
@@ -1469,24 +1524,17 @@ int remc_adjoint(remc_ctx *c, int replica, double *force_torque, double *dparams
     CHECK_REP(c, replica);
     if (!force_torque && !dparams) return fail(REMC_ERR_ARG, "null outputs");
     const HostSys &s = c->systems[c->reps[replica].sys];
-    // scratch: per-chunk bead gradients, bead gradients, per-workgroup parameter
-    // partials, then the per-body outputs and the parameter totals
-    const size_t nd = adjoint_scratch_doubles(c->max_beads, c->max_bodies);
-    if (!c->d_adj) HIP_TRY(hipMalloc(&c->d_adj, nd * sizeof(double)));
-    double *ft = c->d_adj + nd - 6 * (size_t)c->max_bodies - 3, *dp = ft + 6 * (size_t)c->max_bodies;
-    std::vector<unsigned char> af(adjoint_ff_size());
-    adjoint_ff_fill(af.data(), c->ff.table, c->ff.lambda, c->ff.e0, c->ff.xi, c->ff.lj_cf * c->ff.kbt_cf,
-                    c->ff.dh_cf * c->ff.kbt_cf);
-    {
-        TimedLaunch t(c, 4);
-        HIP_TRY(launch_adjoint(c->params(), af.data(), replica, s.n_beads, s.n_bodies, c->max_beads, c->d_adj, ft, dp,
-                               c->stream));
-    }
-    if (force_torque)
-        HIP_TRY(hipMemcpyAsync(force_torque, ft, sizeof(double) * 6 * s.n_bodies, hipMemcpyDeviceToHost, c->stream));
-    if (dparams) HIP_TRY(hipMemcpyAsync(dparams, dp, sizeof(double) * 3, hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return REMC_OK;
+    return adjoint_range(c, replica, 1, s.n_beads, s.n_bodies, c->rep_bead_off[replica], (size_t)s.n_beads,
+                         c->rep_body_off[replica], s.n_bodies, force_torque, dparams);
+}
+
+int remc_adjoint_all(remc_ctx *c, double *force_torque, double *dparams)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (!force_torque && !dparams) return fail(REMC_ERR_ARG, "null outputs");
+    return adjoint_range(c, 0, (int)c->reps.size(), c->max_beads, c->max_bodies, 0, (size_t)c->total_rep_beads, 0,
+                         c->total_rep_bodies, force_torque, dparams);
 }
 
 int remc_timing_enable(remc_ctx *c, int enable)

@@ -111,9 +111,18 @@ hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double
                               int noncrowder_only, int write_E, hipStream_t s);
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
 hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
-hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
-                          int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s);
-size_t adjoint_scratch_doubles(int max_beads, int max_bodies);
+// adjoint launch plan for n_rep replicas of at most max_beads beads whose
+// beads span bead_stride entries of the replica arrays: row tiles, partner
+// sub-ranges and the scratch they need (per-sub-range bead gradients, then
+// per-workgroup parameter partials)
+struct AdjPlan {
+    int tiles, split, sub_len;
+    size_t part_doubles, param_doubles;
+};
+AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride);
+hipError_t launch_adjoint(const KParams &p, const void *adj_ff, const AdjPlan &a, int rep_first, int n_rep,
+                          int max_bodies, int bead_base, size_t bead_stride, int body_base, double *scratch,
+                          double *force_torque, double *dparams, hipStream_t s);
 hipError_t launch_drms(const KParams &p, int sys_id, int n_beads, int n_rep, const float *rx, const float *ry,
                        const float *rz, double *partials, double inv_pairs, double *out, hipStream_t s);
 hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, int n_types,

@@ -30,6 +30,8 @@
 // coefficient dh_cf*kbt_cf, so a pair's value is directly kcal/mol.
 // x^6 is formed by multiplication; exp(-r/xi) = exp2(r*k_dh) with the
 // hardware exp2; 1/r from the hardware reciprocal square root.
+#include <algorithm>
+
 #include "remc_internal.h"
 #include "remc_math.h"
 
@@ -2283,75 +2285,59 @@ hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s)
 }
 
 // --------------------------------------------------------------------------
-// adjoint: gradient of the full energy of one replica
+// adjoint: gradient of the full energy, every replica of a range in one launch
 //
-// k_adjoint_pairs: grid (bead tiles of ADJ_TI) x (partner chunks of N/64
-// beads rounded up to 32, between ADJ_TJ and ADJ_TJ_MAX: about 64 chunks, so
-// that the grid holds several waves per CU); thread i of a tile accumulates, over the chunk's partner beads
-// (staged in LDS), the FP32 pair derivatives (same branches as the pair term)
-// into FP64 dE/dx_i, and the parameter derivatives
-// dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij,
-// dE/dxi = sum DH r / xi^2 over the unordered pairs it owns.  Partials go to
-// per-chunk slots; k_adjoint_reduce adds a bead's chunks in chunk order and
-// k_adjoint_bodies forms per body F = -sum dE/dx_i, the torque sum rel_i x F_i
-// about the body centre, and the parameter totals, so the result does not
-// depend on scheduling.
+// k_adjoint_rows: one workgroup of ADJ_WAVES waves per (row tile of 64 beads,
+// partner sub-range, replica).  The sub-range is staged through LDS in chunks
+// of ADJ_BLOCK beads; wave w takes the staged beads w, w + ADJ_WAVES, ...
+// against its lane's row bead, so every staged value is read as a broadcast.
+// A lane accumulates in FP64, from FP32 pair derivatives (the same branches as
+// the pair term), dE/dx_i of its row bead and the parameter derivatives
+// dE/dlambda = sum dE/dE_ij (T_ij - e0), dE/de0 = -lambda sum dE/dE_ij and
+// dE/dxi = sum DH r / xi^2 over the unordered pairs it owns.  The waves' row
+// gradients are added through LDS in wave order and written once per
+// (sub-range, bead), their parameter sums once per workgroup.
+// k_adjoint_bodies: one wave per (replica, body) adds each bead's sub-range
+// partials in sub-range order, forms F = -sum dE/dx_i and the torque
+// sum rel_i x F_i about the body centre with fixed-order wave sums; the first
+// wave of each replica adds the parameter partials.  The result does not
+// depend on scheduling.  Sub-ranges are only cut when the launch holds fewer
+// than ADJ_TARGET_WG (row tile, replica) workgroups.
 // --------------------------------------------------------------------------
 struct AdjFF {
     float table[REMC_NTYPES * REMC_NTYPES]; // raw table T_ij
     float lambda, e0, xi, c_lj, c_dh;
 };
 
-constexpr int ADJ_TI = 64;       // beads per tile (one wave)
-constexpr int ADJ_TJ = 32;       // smallest partner chunk
-constexpr int ADJ_TJ_MAX = 2048; // largest partner chunk (LDS)
-constexpr int ADJ_CHUNKS = 64;   // partner chunks a system is cut into (fewer when small)
+constexpr int ADJ_ROWS = 64;                 // row beads per workgroup (one per lane)
+constexpr int ADJ_WAVES = 4;                 // waves per workgroup
+constexpr int ADJ_BLOCK = 64 * ADJ_WAVES;    // staged partner beads per chunk
+constexpr int ADJ_TARGET_WG = 2048;          // workgroups a launch aims at
 
-__device__ __host__ inline int adjoint_chunk(int n_beads)
-{
-    int c = (n_beads + ADJ_CHUNKS - 1) / ADJ_CHUNKS;
-    c = ((c + ADJ_TJ - 1) / ADJ_TJ) * ADJ_TJ;
-    return c < ADJ_TJ ? ADJ_TJ : (c > ADJ_TJ_MAX ? ADJ_TJ_MAX : c);
-}
-
-// Most partner chunks of any system of up to max_beads beads: ADJ_CHUNKS
-// (a chunk is at least n/ADJ_CHUNKS beads, or ADJ_TJ when n < ADJ_CHUNKS *
-// ADJ_TJ), more only where the chunk reaches ADJ_TJ_MAX.
-inline size_t adjoint_max_chunks(int max_beads)
-{
-    const size_t big = (size_t)(max_beads + ADJ_TJ_MAX - 1) / ADJ_TJ_MAX;
-    return big > (size_t)ADJ_CHUNKS ? big : (size_t)ADJ_CHUNKS;
-}
-
-__global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, int replica, int chunk,
-                                                          double *part_grad, double *param_part)
+__global__ __launch_bounds__(ADJ_BLOCK) void k_adjoint_rows(KParams p, AdjFF af, int rep_first, int tiles, int split,
+                                                            int sub_len, int bead_base, size_t bead_stride,
+                                                            double *part_grad, double *param_part)
 {
     __shared__ float s_tab[REMC_NTYPES * REMC_NTYPES];
-    extern __shared__ float s_adj[]; // chunk x (x, y, z, q, sigma, type, body)
-    float *s_x = s_adj, *s_y = s_x + chunk, *s_z = s_y + chunk, *s_q = s_z + chunk, *s_s = s_q + chunk;
-    int *s_tc = (int *)(s_s + chunk), *s_body = s_tc + chunk;
-    const int tid = threadIdx.x;
-    const DevRep R = p.rep[replica];
+    __shared__ float s_x[ADJ_BLOCK], s_y[ADJ_BLOCK], s_z[ADJ_BLOCK], s_q[ADJ_BLOCK], s_s[ADJ_BLOCK];
+    __shared__ int s_tc[ADJ_BLOCK], s_body[ADJ_BLOCK];
+    __shared__ double s_g[ADJ_WAVES - 1][3][ADJ_ROWS];
+    __shared__ double s_pp[ADJ_WAVES][3];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tile = blockIdx.x / split, sub = blockIdx.x % split;
+    const int rr = blockIdx.y;
+    const DevRep R = p.rep[rep_first + rr];
     const DevSys S = p.sys[R.sys];
     const DevFF ff = p.ff;
     const int N = S.n_beads;
-    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += ADJ_TI) s_tab[k] = af.table[k];
+    const int j0 = sub * sub_len;
+    if (tile * ADJ_ROWS >= N || j0 >= N) return; // the whole workgroup, before any barrier
+    const int j1 = min(N, j0 + sub_len);
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += ADJ_BLOCK) s_tab[k] = af.table[k];
     const float *gx = p.px + R.bead_off, *gy = p.py + R.bead_off, *gz = p.pz + R.bead_off;
     const int *tcs = p.tc + S.bead_off, *bodies = p.body + S.bead_off;
     const float *qs = p.q + S.bead_off, *sgs = p.sig + S.bead_off;
-    const int i = blockIdx.x * ADJ_TI + tid;
-    const int jc = blockIdx.y, j0 = jc * chunk;
-    const int nj = min(chunk, N - j0);
-    for (int k = tid; k < chunk; k += ADJ_TI) {
-        const int jt = j0 + k;
-        if (k < nj) {
-            s_x[k] = gx[jt]; s_y[k] = gy[jt]; s_z[k] = gz[jt];
-            s_q[k] = qs[jt]; s_s[k] = sgs[jt]; s_tc[k] = tcs[jt]; s_body[k] = bodies[jt];
-        } else {
-            s_body[k] = -2;
-        }
-    }
-    __syncthreads();
+    const int i = tile * ADJ_ROWS + lane;
     const bool vi = i < N;
     const float xi = vi ? gx[i] : 0.f, yi = vi ? gy[i] : 0.f, zi = vi ? gz[i] : 0.f;
     const int tci = vi ? tcs[i] : 0, bi = vi ? bodies[i] : -1;
@@ -2359,106 +2345,142 @@ __global__ __launch_bounds__(ADJ_TI) void k_adjoint_pairs(KParams p, AdjFF af, i
     const bool ci = (tci >> 8) != 0;
     double gxa = 0.0, gya = 0.0, gza = 0.0, dl = 0.0, de0 = 0.0, dxi = 0.0;
     const float inv_xi2 = 1.0f / (af.xi * af.xi);
-    if (vi) {
-        for (int k = 0; k < nj; ++k) {
-            if (s_body[k] == bi) continue;
-            const int tcj = s_tc[k];
-            const bool cj = (tcj >> 8) != 0;
-            const float dx = min_image(xi - s_x[k], S.box, S.inv_box);
-            const float dy = min_image(yi - s_y[k], S.box, S.inv_box);
-            const float dz = min_image(zi - s_z[k], S.box, S.inv_box);
-            const float r2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-36f);
-            const float rinv = __builtin_amdgcn_rsqf(r2);
-            const float r = r2 * rinv;
-            float dedr;
-            if (ci || cj) {
-                const float t = ff.crow_s2 * (rinv * rinv);
-                const float t6 = t * t * t;
-                dedr = r2 < ff.crow_cut2 ? -6.0f * ff.c_crow * t6 * rinv : 0.0f;
-            } else {
-                const int ti = tci & 0xff, tj = tcj & 0xff;
-                const float traw = s_tab[ti * REMC_NTYPES + tj];
-                const float eij = af.lambda * (traw - af.e0);
-                const float sij = 0.5f * (si + s_s[k]);
-                const float s2 = sij * sij * (rinv * rinv);
-                const float x6 = s2 * s2 * s2;
-                const float base = 4.0f * x6 * (x6 - 1.0f);
-                const float dbase = 4.0f * (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv);
-                const bool flip = (eij > 0.0f) & (x6 > 0.5f);
-                const float dlj = flip ? eij * dbase : -eij * dbase;
-                const float dlj_de = flip ? base + 2.0f : -base;
-                const float dh = af.c_dh * qi * s_q[k] * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
-                const float ddh = -dh * (1.0f / af.xi + rinv);
-                dedr = af.c_lj * dlj + ddh;
-                // each unordered pair once for the parameter sums
-                if (s_body[k] > bi) {
-                    const float de_deij = af.c_lj * dlj_de;
-                    dl += (double)(de_deij * (traw - af.e0));
-                    de0 += (double)(-de_deij * af.lambda);
-                    dxi += (double)(dh * r * inv_xi2);
-                }
-            }
-            const float kf = dedr * rinv;
-            gxa += (double)(kf * dx);
-            gya += (double)(kf * dy);
-            gza += (double)(kf * dz);
+    for (int c0 = j0; c0 < j1; c0 += ADJ_BLOCK) {
+        const int nc = min(ADJ_BLOCK, j1 - c0);
+        __syncthreads(); // the previous chunk is consumed (first pass: the table is written)
+        if (tid < nc) {
+            const int jt = c0 + tid;
+            s_x[tid] = gx[jt]; s_y[tid] = gy[jt]; s_z[tid] = gz[jt];
+            s_q[tid] = qs[jt]; s_s[tid] = sgs[jt]; s_tc[tid] = tcs[jt]; s_body[tid] = bodies[jt];
         }
-        double *pg = part_grad + 3 * ((size_t)jc * N + i);
-        pg[0] = gxa;
-        pg[1] = gya;
-        pg[2] = gza;
+        __syncthreads();
+        if (vi) {
+            for (int k = wave; k < nc; k += ADJ_WAVES) {
+                if (s_body[k] == bi) continue;
+                const int tcj = s_tc[k];
+                const bool cj = (tcj >> 8) != 0;
+                const float dx = min_image(xi - s_x[k], S.box, S.inv_box);
+                const float dy = min_image(yi - s_y[k], S.box, S.inv_box);
+                const float dz = min_image(zi - s_z[k], S.box, S.inv_box);
+                const float r2 = fmaxf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), 1e-36f);
+                const float rinv = __builtin_amdgcn_rsqf(r2);
+                const float r = r2 * rinv;
+                float dedr;
+                if (ci || cj) {
+                    const float t = ff.crow_s2 * (rinv * rinv);
+                    const float t6 = t * t * t;
+                    dedr = r2 < ff.crow_cut2 ? -6.0f * ff.c_crow * t6 * rinv : 0.0f;
+                } else {
+                    const int ti = tci & 0xff, tj = tcj & 0xff;
+                    const float traw = s_tab[ti * REMC_NTYPES + tj];
+                    const float eij = af.lambda * (traw - af.e0);
+                    const float sij = 0.5f * (si + s_s[k]);
+                    const float s2 = sij * sij * (rinv * rinv);
+                    const float x6 = s2 * s2 * s2;
+                    const float base = 4.0f * x6 * (x6 - 1.0f);
+                    const float dbase = 4.0f * (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv);
+                    const bool flip = (eij > 0.0f) & (x6 > 0.5f);
+                    const float dlj = flip ? eij * dbase : -eij * dbase;
+                    const float dlj_de = flip ? base + 2.0f : -base;
+                    const float dh = af.c_dh * qi * s_q[k] * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+                    const float ddh = -dh * (1.0f / af.xi + rinv);
+                    dedr = af.c_lj * dlj + ddh;
+                    // each unordered pair once for the parameter sums
+                    if (s_body[k] > bi) {
+                        const float de_deij = af.c_lj * dlj_de;
+                        dl += (double)(de_deij * (traw - af.e0));
+                        de0 += (double)(-de_deij * af.lambda);
+                        dxi += (double)(dh * r * inv_xi2);
+                    }
+                }
+                const float kf = dedr * rinv;
+                gxa += (double)(kf * dx);
+                gya += (double)(kf * dy);
+                gza += (double)(kf * dz);
+            }
+        }
+    }
+    if (wave > 0) {
+        s_g[wave - 1][0][lane] = gxa;
+        s_g[wave - 1][1][lane] = gya;
+        s_g[wave - 1][2][lane] = gza;
     }
     const double v0 = wave_sum_dpp(dl), v1 = wave_sum_dpp(de0), v2 = wave_sum_dpp(dxi);
-    if (tid == 0) {
-        double *pp = param_part + 3 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
-        pp[0] = v0;
-        pp[1] = v1;
-        pp[2] = v2;
+    if (lane == 0) {
+        s_pp[wave][0] = v0;
+        s_pp[wave][1] = v1;
+        s_pp[wave][2] = v2;
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int w = 0; w < ADJ_WAVES - 1; ++w) {
+            gxa += s_g[w][0][lane];
+            gya += s_g[w][1][lane];
+            gza += s_g[w][2][lane];
+        }
+        if (vi) {
+            double *pg = part_grad + 3 * ((size_t)sub * bead_stride + (size_t)(R.bead_off - bead_base) + i);
+            pg[0] = gxa;
+            pg[1] = gya;
+            pg[2] = gza;
+        }
+        if (lane < 3) {
+            double t = 0.0;
+#pragma unroll
+            for (int w = 0; w < ADJ_WAVES; ++w) t += s_pp[w][lane];
+            param_part[3 * (((size_t)rr * tiles + tile) * split + sub) + lane] = t;
+        }
     }
 }
 
-__global__ void k_adjoint_reduce(int n_beads, int n_chunks, const double *part_grad, double *bead_grad)
+// One wave per (replica, body): the lanes stride over the body's beads, then
+// wave sums (fixed order, so every call gives the same bits); the first wave of
+// workgroup (0, replica) also adds the replica's parameter partials.
+__global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int rep_first, int tiles, int split, int sub_len,
+                                                        int bead_base, size_t bead_stride, int body_base,
+                                                        const double *part_grad, const double *param_part,
+                                                        double *force_torque, double *dparams)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * n_beads) return;
-    double g = 0.0;
-    for (int c = 0; c < n_chunks; ++c) g += part_grad[(size_t)c * 3 * n_beads + t];
-    bead_grad[t] = g;
-}
-
-// One wave per body: the lanes stride over its beads, then wave sums (fixed
-// order, so every call gives the same bits); the first wave of workgroup 0 also
-// adds the parameter partials.
-__global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int replica, const double *bead_grad,
-                                                        const double *param_part, int n_parts, double *force_torque,
-                                                        double *dparams)
-{
-    const DevRep R = p.rep[replica];
+    const int rr = blockIdx.y;
+    const DevRep R = p.rep[rep_first + rr];
     const DevSys S = p.sys[R.sys];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int N = S.n_beads;
+    const int nsub = (N + sub_len - 1) / sub_len, ntile = (N + ADJ_ROWS - 1) / ADJ_ROWS;
     if (blockIdx.x == 0 && wave == 0) {
         double t0 = 0.0, t1 = 0.0, t2 = 0.0;
-        for (int a = lane; a < n_parts; a += 64) {
-            t0 += param_part[3 * a];
-            t1 += param_part[3 * a + 1];
-            t2 += param_part[3 * a + 2];
+        for (int a = lane; a < ntile * nsub; a += 64) {
+            const int tile = a / nsub, sub = a - tile * nsub;
+            const double *pp = param_part + 3 * (((size_t)rr * tiles + tile) * split + sub);
+            t0 += pp[0];
+            t1 += pp[1];
+            t2 += pp[2];
         }
         t0 = wave_sum_dpp(t0);
         t1 = wave_sum_dpp(t1);
         t2 = wave_sum_dpp(t2);
         if (lane == 0) {
-            dparams[0] = t0;
-            dparams[1] = t1;
-            dparams[2] = t2;
+            dparams[3 * rr + 0] = t0;
+            dparams[3 * rr + 1] = t1;
+            dparams[3 * rr + 2] = t2;
         }
     }
     const int m = blockIdx.x * (blockDim.x >> 6) + wave;
     if (m >= S.n_bodies) return;
     const int b0 = p.bstart[S.body_off + m], b1 = p.bend[S.body_off + m];
     const float *rx = p.rx + R.bead_off, *ry = p.ry + R.bead_off, *rz = p.rz + R.bead_off;
+    const double *pg = part_grad + 3 * (size_t)(R.bead_off - bead_base);
     double F[3] = {0.0, 0.0, 0.0}, T[3] = {0.0, 0.0, 0.0};
     for (int i = b0 + lane; i < b1; i += 64) {
-        const double f0 = -bead_grad[3 * i], f1 = -bead_grad[3 * i + 1], f2 = -bead_grad[3 * i + 2];
+        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+        for (int s = 0; s < nsub; ++s) {
+            const double *q = pg + 3 * ((size_t)s * bead_stride + i);
+            g0 += q[0];
+            g1 += q[1];
+            g2 += q[2];
+        }
+        const double f0 = -g0, f1 = -g1, f2 = -g2;
         F[0] += f0; F[1] += f1; F[2] += f2;
         const double ax = rx[i], ay = ry[i], az = rz[i];
         T[0] += ay * f2 - az * f1;
@@ -2469,38 +2491,41 @@ __global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int replica, 
         F[k] = wave_sum_dpp(F[k]);
         T[k] = wave_sum_dpp(T[k]);
     }
-    if (lane == 0)
+    if (lane == 0) {
+        double *o = force_torque + 6 * ((size_t)(R.body_off - body_base) + m);
         for (int k = 0; k < 3; ++k) {
-            force_torque[6 * m + k] = F[k];
-            force_torque[6 * m + 3 + k] = T[k];
+            o[k] = F[k];
+            o[3 + k] = T[k];
         }
+    }
 }
 
-size_t adjoint_scratch_doubles(int max_beads, int max_bodies)
+AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride)
 {
-    const size_t ti = (size_t)(max_beads + ADJ_TI - 1) / ADJ_TI, tj = adjoint_max_chunks(max_beads);
-    return 3 * tj * (size_t)max_beads + 3 * (size_t)max_beads + 3 * ti * tj + 6 * (size_t)max_bodies + 3;
+    AdjPlan a{};
+    a.tiles = (max_beads + ADJ_ROWS - 1) / ADJ_ROWS;
+    const long long units = (long long)n_rep * a.tiles;
+    const long long want = units >= ADJ_TARGET_WG ? 1 : (ADJ_TARGET_WG + units - 1) / units;
+    const long long per = (max_beads + want - 1) / want;
+    a.sub_len = (int)std::max<long long>(ADJ_ROWS, (per + ADJ_ROWS - 1) / ADJ_ROWS * ADJ_ROWS);
+    a.split = (max_beads + a.sub_len - 1) / a.sub_len;
+    a.part_doubles = 3 * (size_t)a.split * bead_stride;
+    a.param_doubles = 3 * (size_t)n_rep * a.tiles * a.split;
+    return a;
 }
 
-hipError_t launch_adjoint(const KParams &p, const void *adj_ff, int replica, int n_beads, int n_bodies,
-                          int max_beads, double *scratch, double *force_torque, double *dparams, hipStream_t s)
+hipError_t launch_adjoint(const KParams &p, const void *adj_ff, const AdjPlan &a, int rep_first, int n_rep,
+                          int max_bodies, int bead_base, size_t bead_stride, int body_base, double *scratch,
+                          double *force_torque, double *dparams, hipStream_t s)
 {
-    // every system of the context has at most adjoint_max_chunks(max_beads) chunks: the scratch's stride
-    const int chunk = adjoint_chunk(n_beads);
-    const int ti = (n_beads + ADJ_TI - 1) / ADJ_TI, tj = (n_beads + chunk - 1) / chunk;
-    const size_t mtj = adjoint_max_chunks(max_beads);
-    double *part_grad = scratch, *bead_grad = part_grad + 3 * mtj * (size_t)max_beads;
-    double *param_part = bead_grad + 3 * (size_t)max_beads;
+    if (n_rep <= 0) return hipSuccess;
+    double *part_grad = scratch, *param_part = scratch + a.part_doubles;
     const AdjFF af = *(const AdjFF *)adj_ff;
-    const size_t lds = 28 * (size_t)chunk;
-    hipError_t e = hipFuncSetAttribute((const void *)k_adjoint_pairs, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_adjoint_pairs, dim3(ti, tj), dim3(ADJ_TI), lds, s, p, af, replica, chunk, part_grad,
-                       param_part);
-    hipLaunchKernelGGL(k_adjoint_reduce, dim3((3 * n_beads + 255) / 256), dim3(256), 0, s, n_beads, tj, part_grad,
-                       bead_grad);
-    hipLaunchKernelGGL(k_adjoint_bodies, dim3((n_bodies + 3) / 4), dim3(256), 0, s, p, replica, bead_grad, param_part,
-                       ti * tj, force_torque, dparams);
+    hipLaunchKernelGGL(k_adjoint_rows, dim3(a.tiles * a.split, n_rep), dim3(ADJ_BLOCK), 0, s, p, af, rep_first,
+                       a.tiles, a.split, a.sub_len, bead_base, bead_stride, part_grad, param_part);
+    hipLaunchKernelGGL(k_adjoint_bodies, dim3((max_bodies + 3) / 4, n_rep), dim3(256), 0, s, p, rep_first, a.tiles,
+                       a.split, a.sub_len, bead_base, bead_stride, body_base, part_grad, param_part, force_torque,
+                       dparams);
     return hipGetLastError();
 }
 

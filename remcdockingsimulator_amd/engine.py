@@ -180,6 +180,19 @@ class Engine:
         self._chk(self.lib.remc_adjoint(self.ctx, replica, nat.dptr(ft), nat.dptr(dp)), "remc_adjoint")
         return ft, dp
 
+    def adjoint_all(self):
+        """Forces/torques (list of (m,6) per replica) and dE/d(lambda,e0,xi)
+        (R,3) of every replica from one batched launch."""
+        sizes = [self.systems[s].n_bodies for s in self.replica_sys]
+        ft = np.zeros((sum(sizes), 6), dtype=np.float64)
+        dp = np.zeros((self.n_replicas, 3), dtype=np.float64)
+        self._chk(self.lib.remc_adjoint_all(self.ctx, nat.dptr(ft), nat.dptr(dp)), "remc_adjoint_all")
+        out, o = [], 0
+        for m in sizes:
+            out.append(ft[o:o + m])
+            o += m
+        return out, dp
+
     # ----------------------------------------------------------- state ----
     def energies(self):
         out = np.zeros(self.n_replicas, dtype=np.float64)

@@ -1,5 +1,6 @@
-"""GPU adjoint (remc_adjoint) against the oracle's FD-pinned analytic
-gradient, and a torch.autograd.Function built on it (check harness only).
+"""GPU adjoint (remc_adjoint, remc_adjoint_all) against the oracle's
+FD-pinned analytic gradient, and a torch.autograd.Function built on it
+(check harness only).
 
 Bar: per-body forces/torques within 1e-4 of the largest component and
 parameter gradients within 1e-4 relative (FP32 pair derivatives, FP64
@@ -30,8 +31,8 @@ def check(ft_gpu, dp_gpu, ft, dp):
 
 @pytest.mark.parametrize("cid", [1, 2, 4, 5])
 def test_adjoint_matches_oracle_configs(oracle, cid):
-    """Config 5 (11000 beads) takes the kernel's large-chunk path (partner
-    chunks of N/16 beads)."""
+    """One replica per call: the partner range is cut into sub-ranges of 64
+    beads on configs 1-4 and 960 beads on config 5 (11000 beads)."""
     wl = synth.config(cid)
     sa = wl.systems[0]
     e = engine_for(wl, seed=3)
@@ -100,4 +101,57 @@ def test_autograd_function_translations(oracle):
             p2[body == m, k] -= 1e-4
             fd = (osys.energy_d(p1) - osys.energy_d(p2)) / 2e-4
             assert abs(t.grad[m, k].item() - fd) <= 1e-4 * max(abs(fd), 1e-3), (m, k, t.grad[m, k].item(), fd)
+    e.close()
+
+
+@pytest.mark.parametrize("cid", [1, 4, 5])
+def test_adjoint_all_matches_oracle_and_single_calls(oracle, cid):
+    """Every replica in one launch (config 4: 4 partner sub-ranges per row
+    tile; config 5: none) against the oracle for sampled replicas and against
+    the one-replica calls (same pair arithmetic, other sub-range cuts) for all."""
+    wl = synth.config(cid)
+    sa = wl.systems[0]
+    e = engine_for(wl, seed=4)
+    e.mc_steps(150)
+    e.synchronize()
+    say("adjoint_all config", cid)
+    fts, dps = e.adjoint_all()
+    assert len(fts) == wl.n_replicas and dps.shape == (wl.n_replicas, 3)
+    for r in range(wl.n_replicas):
+        ft1, dp1 = e.adjoint(r)
+        scale = np.max(np.abs(ft1))
+        assert np.max(np.abs(fts[r] - ft1)) <= 1e-9 * scale + 1e-12, (r, np.max(np.abs(fts[r] - ft1)) / scale)
+        assert np.allclose(dps[r], dp1, rtol=1e-9, atol=1e-12), (r, dps[r], dp1)
+    for r in ((0, wl.n_replicas // 2, wl.n_replicas - 1) if cid != 5 else (0, 63)):
+        ft, dp = oracle_ft(oracle, sa, wl.ff, e.state(r))
+        check(fts[r], dps[r], ft, dp)
+    # a second call gives the same bits (fixed-order sums)
+    fts2, dps2 = e.adjoint_all()
+    assert all(np.array_equal(a, b) for a, b in zip(fts, fts2)) and np.array_equal(dps, dps2)
+    e.close()
+
+
+def test_adjoint_all_ragged_systems(oracle):
+    """Systems of different sizes in one context: each replica's rows, bodies
+    and outputs land at its own offsets."""
+    from remcdockingsimulator_amd import forcefield as ffm
+
+    ff = ffm.ForceField.synthetic(9)
+    systems = [synth.small_system(s, n_chains=nc, chain_len=(8, 40), n_crowders=ncr, box=70.0)
+               for s, nc, ncr in ((1, 2, 3), (2, 5, 40), (3, 3, 0))]
+    e = Engine(0)
+    e.set_forcefield(ff)
+    e.set_seed(11)
+    reps = []
+    for k, sa in enumerate(systems):
+        sid = e.add_system(sa)
+        e.add_replicas(sid, [290.0 + 10 * j for j in range(k + 2)], 2.0, 0.2)
+        reps += [sa] * (k + 2)
+    e.mc_steps(100)
+    e.synchronize()
+    fts, dps = e.adjoint_all()
+    for r, sa in enumerate(reps):
+        assert fts[r].shape == (sa.n_bodies, 6)
+        ft, dp = oracle_ft(oracle, sa, ff, e.state(r))
+        check(fts[r], dps[r], ft, dp)
     e.close()
