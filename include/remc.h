@@ -129,6 +129,18 @@ REMC_API int remc_energy_plain(remc_ctx *ctx, int replica, int block, double *ou
  * Kahan sums per thread, FP64 across threads and tiles.  out[n_replicas].
  * Same quantity as Replica::E() (Replica.cpp:603-745). */
 REMC_API int remc_energy_all(remc_ctx *ctx, double *out);
+/* remc_energy_all without any allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Its kernels' partials live
+ * in a static 4 MiB device array per device instead of the context's buffers:
+ * at most 524,288 doubles, of which it needs R (4 U + 1) for R replicas,
+ * U = G (G/2 + 1) and G = ceil(N/256) for the largest system's bead count N
+ * (config 5, 64 replicas of 11,000 beads: 242,240); larger contexts get
+ * REMC_ERR_LIMIT.
+ * The context must already be set up by an earlier call (any energy, state or
+ * MC call), else REMC_ERR_STATE.  Every context on one device shares the
+ * array: calls that overlap on one device race. */
+REMC_API int remc_energy_all_noalloc(remc_ctx *ctx, double *out);
 /* Sets every replica's tracked energy to remc_energy_all's value. */
 REMC_API int remc_refresh_energies(remc_ctx *ctx);
 /* Energy of pairs between non-crowder bodies only, all replicas; the

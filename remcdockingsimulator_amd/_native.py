@@ -60,6 +60,7 @@ SIGNATURES = {
     "remc_get_sizes": (ctypes.c_int, [_vp, _ip, _ip, _ip]),
     "remc_energy_plain": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp]),
     "remc_energy_all": (ctypes.c_int, [_vp, _dp]),
+    "remc_energy_all_noalloc": (ctypes.c_int, [_vp, _dp]),
     "remc_refresh_energies": (ctypes.c_int, [_vp]),
     "remc_energy_noncrowder_all": (ctypes.c_int, [_vp, _dp]),
     "remc_mc_steps": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -77,6 +78,38 @@ hipError_t upload(T *d, const std::vector<T> &h, hipStream_t s)
 
 } // namespace
 
+// Scratch of remc_energy_all_noalloc: the full-energy kernels' FP64 unit
+// partials and per-replica totals, in device memory the library never
+// allocates (4 MiB; each device the library runs on holds its copy until the
+// process ends).  In host code `scratch` is a stand-in for the symbol, not a
+// device address: its address on a device comes from hipGetSymbolAddress.
+constexpr size_t REMC_NOALLOC_DOUBLES = 524288;
+static __device__ __align__(256) double scratch[REMC_NOALLOC_DOUBLES];
+
+namespace {
+
+constexpr int REMC_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[REMC_MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current).
+hipError_t scratch_address(int device, double **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= REMC_MAX_DEVICES) return hipErrorInvalidDevice;
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = (double *)ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+} // namespace
+
 struct remc_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -118,6 +151,7 @@ struct remc_ctx {
     void *d_ana = nullptr;   // analysis scratch (reference conformation, partials, histogram)
     size_t ana_bytes = 0;
     DevFF devff{};
+    double *noalloc_scratch = nullptr; // this device's `scratch` (remc_energy_all_noalloc)
 
     // MC configuration: workgroup size and workgroups per replica (0 = autotune)
     int mc_block = 0;
@@ -741,6 +775,11 @@ int remc_ctx_create(int device, remc_ctx **out)
     HIP_TRY(hipSetDevice(device));
     remc_ctx *c = new remc_ctx();
     c->device = device;
+    e = scratch_address(device, &c->noalloc_scratch);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(REMC_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    }
     e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -938,6 +977,29 @@ int remc_energy_all(remc_ctx *c, double *out)
         HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(out, c->d_out, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_energy_all_noalloc(remc_ctx *c, double *out)
+{
+    CHECK_CTX(c);
+    if (!out) return fail(REMC_ERR_ARG, "null output");
+    // setting a context up allocates its buffers: that must have happened in
+    // an earlier call, since this one allocates nothing
+    if (!c->finalized)
+        return fail(REMC_ERR_STATE, "context not set up: make a call that sets it up first (remc_get_energies, "
+                                    "remc_energy_all, ...); remc_energy_all_noalloc allocates nothing");
+    if (!c->noalloc_scratch) return fail(REMC_ERR_STATE, "no scratch address for device %d", c->device);
+    const int R = (int)c->reps.size();
+    const size_t stride = (size_t)full_units(c->max_tiles) * FULL_SPLIT_MAX;
+    const size_t need = stride * (size_t)R + (size_t)R;
+    if (need > REMC_NOALLOC_DOUBLES)
+        return fail(REMC_ERR_LIMIT, "remc_energy_all_noalloc needs %zu doubles of scratch for %d replicas of up to %d "
+                                    "beads; its static scratch holds %zu", need, R, c->max_beads, REMC_NOALLOC_DOUBLES);
+    double *partials = c->noalloc_scratch, *dout = partials + stride * (size_t)R;
+    HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, partials, dout, 0, 0, c->stream));
+    HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
 }

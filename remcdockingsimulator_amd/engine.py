@@ -109,6 +109,13 @@ class Engine:
         self._chk(self.lib.remc_energy_all(self.ctx, nat.dptr(out)), "remc_energy_all")
         return out
 
+    def energy_all_noalloc(self):
+        """remc_energy_all_noalloc: the same energies from the library's static
+        scratch (the context must be set up by an earlier call)."""
+        out = np.zeros(self.n_replicas, dtype=np.float64)
+        self._chk(self.lib.remc_energy_all_noalloc(self.ctx, nat.dptr(out)), "remc_energy_all_noalloc")
+        return out
+
     def energy_noncrowder_all(self):
         out = np.zeros(self.n_replicas, dtype=np.float64)
         self._chk(self.lib.remc_energy_noncrowder_all(self.ctx, nat.dptr(out)), "remc_energy_noncrowder_all")
