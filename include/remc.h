@@ -174,7 +174,9 @@ REMC_API int remc_mc_steps_plain(remc_ctx *ctx, int replica, int nsteps, int blo
  * move is split over them.  Runs of one-bead crowder moves are taken in
  * windows (one per wave) when the replica fits in LDS, switched with
  * remc_set_mc_window; decisions and states equal those of one step at a time
- * up to rounding.  block: workgroup size (256, 512 or 1024; 0 = 512). */
+ * up to rounding.  block: workgroup size (256, 512 or 1024; 0 = the size an
+ * autotuner picks on first use: 128 steps at each size on the live state,
+ * which is then restored bit for bit). */
 REMC_API int remc_mc_steps_fb(remc_ctx *ctx, int nsteps, int block);
 /* One replica-exchange round over every system's temperature ladder, pairs
  * (k,k+1) for k = offset, offset+2, ...; the offset alternates per call
@@ -185,6 +187,15 @@ REMC_API int remc_exchange(remc_ctx *ctx, int *n_swaps);
  * hipGraph and replayed.  Replaces the REMCSimulation/MCthreadableFunction
  * lock-step loop (main.cpp:971-1012, 1378-1469). */
 REMC_API int remc_run(remc_ctx *ctx, int rounds, int steps_per_round);
+/* enable = 1: every round remc_run captures re-evaluates every replica's
+ * energy (as remc_refresh_energies) between the MC launch and the exchange,
+ * so the exchanges read full energies as the reference's do
+ * (main.cpp:1427-1453: its potential is re-evaluated every step).  Default 0:
+ * the exchanges read the tracked energies, the FP64 sums of the accepted
+ * steps' FP32 dE, which differ from the full evaluation by the pair terms'
+ * rounding (about 1e-6 kT after a 20,000-step run of config 4).  No
+ * counterpart in the reference. */
+REMC_API int remc_set_run_refresh(remc_ctx *ctx, int enable);
 /* Records per-step decisions for the first max_steps steps after this call
  * (0 disables). */
 REMC_API int remc_set_recording(remc_ctx *ctx, int max_steps);
@@ -260,6 +271,9 @@ REMC_API int remc_synchronize(remc_ctx *ctx);
  * 256, 512 or 1024). */
 REMC_API int remc_set_mc_block(remc_ctx *ctx, int block);
 REMC_API int remc_get_mc_block(remc_ctx *ctx, int *block);
+/* The force-biased kernel's autotuned workgroup size (0 until remc_mc_steps_fb
+ * has been called with block 0). */
+REMC_API int remc_get_fb_block(remc_ctx *ctx, int *block);
 /* Workgroups per replica of the MC kernel (0 = autotune; else 1, 2, 4 or 8).
  * With more than one, each workgroup keeps a private copy of its replica's
  * state, evaluates dE over a slice of the partner beads, and the members add

@@ -68,6 +68,7 @@ SIGNATURES = {
     "remc_mc_steps_fb": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
     "remc_exchange": (ctypes.c_int, [_vp, _ip]),
     "remc_run": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int]),
+    "remc_set_run_refresh": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_set_recording": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_get_records": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(remc_step_record), ctypes.c_int]),
     "remc_get_energies": (ctypes.c_int, [_vp, _dp]),
@@ -91,6 +92,7 @@ SIGNATURES = {
     "remc_timing_get": (ctypes.c_int, [_vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int64)]),
     "remc_synchronize": (ctypes.c_int, [_vp]),
     "remc_set_mc_block": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "remc_get_fb_block": (ctypes.c_int, [_vp, _ip]),
     "remc_get_mc_block": (ctypes.c_int, [_vp, _ip]),
     "remc_set_mc_coop": (ctypes.c_int, [_vp, ctypes.c_int]),
     "remc_get_mc_coop": (ctypes.c_int, [_vp, _ip]),

@@ -11,6 +11,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -152,6 +153,9 @@ struct remc_ctx {
     size_t ana_bytes = 0;
     DevFF devff{};
     double *noalloc_scratch = nullptr; // this device's `scratch` (remc_energy_all_noalloc)
+    int full_split = 0;      // full-energy partner sub-ranges per unit (0: the size rule; set at set-up)
+    int fb_block = 0;        // force-biased kernel's workgroup size (0 = autotune on first use)
+    bool run_refresh = false; // remc_run: energy refresh between the MC launch and the exchange
 
     // MC configuration: workgroup size and workgroups per replica (0 = autotune)
     int mc_block = 0;
@@ -268,6 +272,78 @@ void destroy_graph(remc_ctx *c)
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
     c->graph_steps = -1;
+}
+
+// Two timing events, destroyed with the holder.
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t create()
+    {
+        hipError_t e = hipEventCreate(&a);
+        return e != hipSuccess ? e : hipEventCreate(&b);
+    }
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// Partner split of the full-energy kernels for each (replicas, bead tiles):
+// timed once per process and grid size, so every context of a process with
+// that size takes the same split and their energies agree bit for bit.
+std::mutex g_split_mu;
+std::map<std::pair<int, int>, int> g_split_cache;
+
+int autotune_energy(remc_ctx *c)
+{
+    const int R = (int)c->reps.size();
+    const std::pair<int, int> key(R, c->max_tiles);
+    {
+        std::lock_guard<std::mutex> lock(g_split_mu);
+        auto it = g_split_cache.find(key);
+        if (it != g_split_cache.end()) {
+            c->full_split = it->second;
+            return REMC_OK;
+        }
+    }
+    // each candidate on the live conformations (energies only, nothing
+    // written back), best of three; the size rule's choice is kept unless
+    // another split is at least 3% faster
+    const int def = full_split_default(R, c->max_tiles);
+    const int cand[3] = {1, 2, FULL_SPLIT_MAX};
+    float best_ms[3] = {0.f, 0.f, 0.f};
+    EventPair ev;
+    HIP_TRY(ev.create());
+    const KParams p = c->params();
+    for (int k = 0; k < 3; ++k) {
+        HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, cand[k], c->stream));
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_TRY(hipEventRecord(ev.a, c->stream));
+            HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, cand[k], c->stream));
+            HIP_TRY(hipEventRecord(ev.b, c->stream));
+            HIP_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            best = std::min(best, ms);
+        }
+        best_ms[k] = best;
+    }
+    float t_def = 1e30f;
+    for (int k = 0; k < 3; ++k)
+        if (cand[k] == def) t_def = best_ms[k];
+    int pick = def;
+    float t_pick = t_def;
+    for (int k = 0; k < 3; ++k)
+        if (best_ms[k] < 0.97f * t_def && best_ms[k] < t_pick) {
+            pick = cand[k];
+            t_pick = best_ms[k];
+        }
+    std::lock_guard<std::mutex> lock(g_split_mu);
+    auto ins = g_split_cache.emplace(key, pick); // a context set up meanwhile keeps its choice
+    c->full_split = ins.first->second;
+    return REMC_OK;
 }
 
 int finalize(remc_ctx *c)
@@ -482,9 +558,9 @@ int finalize(remc_ctx *c)
 
     // initial energies of every replica
     KParams p = c->params();
-    HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, s));
+    HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, 0, s));
     HIP_TRY(hipStreamSynchronize(s));
-    return REMC_OK;
+    return autotune_energy(c);
 }
 
 #define CHECK_CTX(c)                                                   \
@@ -617,63 +693,72 @@ int mc_launch(remc_ctx *c, const KParams &p0, int nsteps, int block, int coop)
     return REMC_OK;
 }
 
-int autotune_mc(remc_ctx *c)
-{
-    // time each (workgroup size, workgroups per replica) on the live state, then restore it exactly
-    const int R = (int)c->reps.size();
+// The copy-0 dynamic state of every replica (positions, relative positions,
+// centres, rotations, tracked energies, step and decision counters), saved
+// so that trial launches can run on the live state.  restore() puts it back
+// bit for bit; the destructor restores it once more on every exit (error
+// paths included) and releases the saves.
+struct StateGuard {
     struct Buf {
         void *dev;
         size_t bytes;
         void *save;
     };
-    // Owns the saved state and the events: on every exit, the error paths
-    // included, the live state is put back from the saves (a no-op after a
-    // completed trial), then the saves and events are released.
-    struct Trials {
-        remc_ctx *c;
-        std::vector<Buf> bufs;
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        ~Trials()
-        {
-            for (auto &b : bufs)
-                if (b.save) (void)hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream);
-            (void)hipStreamSynchronize(c->stream);
-            for (auto &b : bufs)
-                if (b.save) (void)hipFree(b.save);
-            if (e0) (void)hipEventDestroy(e0);
-            if (e1) (void)hipEventDestroy(e1);
-            c->synced_copies = 1;
-        }
-    };
-    int rc = sync_copies(c, 1);
-    if (rc != REMC_OK) return rc;
-    Trials tr{c,
-              {{c->d_px, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_py, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_pz, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_rx, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_ry, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_rz, sizeof(float) * c->total_rep_beads, nullptr},
-               {c->d_cx, sizeof(float) * c->total_rep_bodies, nullptr},
-               {c->d_cy, sizeof(float) * c->total_rep_bodies, nullptr},
-               {c->d_cz, sizeof(float) * c->total_rep_bodies, nullptr},
-               {c->d_quat, sizeof(double) * 4 * c->total_rep_bodies, nullptr},
-               {c->d_E, sizeof(double) * R, nullptr},
-               {c->d_step, sizeof(unsigned long long) * R, nullptr},
-               {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}}};
-    for (auto &b : tr.bufs) {
-        void *save = nullptr;
-        HIP_TRY(hipMalloc(&save, b.bytes));
-        b.save = save; // set only once allocated: the guard copies back from it
-        HIP_TRY(hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+    remc_ctx *c;
+    std::vector<Buf> bufs;
+    explicit StateGuard(remc_ctx *c_) : c(c_)
+    {
+        const int R = (int)c->reps.size();
+        const size_t nb = (size_t)c->total_rep_beads, nm = (size_t)c->total_rep_bodies;
+        bufs = {{c->d_px, sizeof(float) * nb, nullptr},          {c->d_py, sizeof(float) * nb, nullptr},
+                {c->d_pz, sizeof(float) * nb, nullptr},          {c->d_rx, sizeof(float) * nb, nullptr},
+                {c->d_ry, sizeof(float) * nb, nullptr},          {c->d_rz, sizeof(float) * nb, nullptr},
+                {c->d_cx, sizeof(float) * nm, nullptr},          {c->d_cy, sizeof(float) * nm, nullptr},
+                {c->d_cz, sizeof(float) * nm, nullptr},          {c->d_quat, sizeof(double) * 4 * nm, nullptr},
+                {c->d_E, sizeof(double) * R, nullptr},           {c->d_step, sizeof(unsigned long long) * R, nullptr},
+                {c->d_cnt, sizeof(unsigned long long) * 3 * R, nullptr}};
     }
+    int save()
+    {
+        int rc = sync_copies(c, 1);
+        if (rc != REMC_OK) return rc;
+        for (auto &b : bufs) {
+            void *ptr = nullptr;
+            HIP_TRY(hipMalloc(&ptr, b.bytes));
+            b.save = ptr; // set only once allocated: the destructor copies back from it
+            HIP_TRY(hipMemcpyAsync(b.save, b.dev, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+        }
+        return REMC_OK;
+    }
+    int restore()
+    {
+        for (auto &b : bufs)
+            if (b.save) HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->synced_copies = 1;
+        return REMC_OK;
+    }
+    ~StateGuard()
+    {
+        (void)restore();
+        (void)hipStreamSynchronize(c->stream);
+        for (auto &b : bufs)
+            if (b.save) (void)hipFree(b.save);
+    }
+};
+
+int autotune_mc(remc_ctx *c)
+{
+    // time each (workgroup size, workgroups per replica) on the live state, then restore it exactly
+    StateGuard st(c);
+    int rc = st.save();
+    if (rc != REMC_OK) return rc;
+    EventPair ev;
+    HIP_TRY(ev.create());
     KParams p = c->params();
     p.rec = nullptr;
     const int blocks[3] = {256, 512, 1024};
     double best = 1e30;
     int best_block = 256, best_coop = 1;
-    HIP_TRY(hipEventCreate(&tr.e0));
-    HIP_TRY(hipEventCreate(&tr.e1));
     const int trial_steps = 256;
     for (int bi = 0; bi < 3; ++bi) {
         const int blk = blocks[bi];
@@ -690,18 +775,18 @@ int autotune_mc(remc_ctx *c)
                 c->synced_copies = 1;
                 rc = mc_launch(c, p, 4, blk, coop); // warm
                 if (rc != REMC_OK) return rc;
-                HIP_TRY(hipEventRecord(tr.e0, c->stream));
+                HIP_TRY(hipEventRecord(ev.a, c->stream));
                 rc = mc_launch(c, p, trial_steps, blk, coop);
                 if (rc != REMC_OK) return rc;
-                HIP_TRY(hipEventRecord(tr.e1, c->stream));
-                HIP_TRY(hipEventSynchronize(tr.e1));
+                HIP_TRY(hipEventRecord(ev.b, c->stream));
+                HIP_TRY(hipEventSynchronize(ev.b));
                 rc = check_err(c);
                 if (rc != REMC_OK) return rc;
                 float ms = 0.f;
-                HIP_TRY(hipEventElapsedTime(&ms, tr.e0, tr.e1));
+                HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
                 ms_best = std::min(ms_best, ms);
-                for (auto &b : tr.bufs)
-                    HIP_TRY(hipMemcpyAsync(b.dev, b.save, b.bytes, hipMemcpyDeviceToDevice, c->stream));
+                rc = st.restore();
+                if (rc != REMC_OK) return rc;
             }
             if (ms_best < best) {
                 best = ms_best;
@@ -712,6 +797,92 @@ int autotune_mc(remc_ctx *c)
     }
     c->mc_block = best_block;
     c->mc_coop = best_coop;
+    return REMC_OK;
+}
+
+// Launch set-up of the force-biased kernel at one workgroup size: the LDS
+// layout (the replica's partner beads with their relative positions, and its
+// body centres and rotations, cached when they all fit) and the workgroups
+// per replica.
+int fb_prepare(remc_ctx *c, int block, KParams *out, int *coop_out)
+{
+    KParams p = c->params();
+    p.lds_mb = std::max(1, c->max_body_beads);
+    p.lds_cl = 0;
+    const long long static_lds = 13 * 1024; // table, reductions, exchange, 64 batched draws
+    if ((long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block)
+        return fail(REMC_ERR_LIMIT, "a body of %d beads does not fit the force-biased kernel's LDS staging",
+                    c->max_body_beads);
+    p.lds_cl = c->max_beads;
+    p.lds_nbody = c->max_bodies;
+    if (!c->lds_cache || (long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block) {
+        p.lds_cl = 0;
+        p.lds_nbody = 0;
+    }
+    // windows of one-bead crowder moves (with the cache), switched with the MC kernel's
+    p.spec = c->mc_window ? 1 : 0;
+    // workgroups per replica: the MC kernel's setting when one is made, else
+    // as many as give every workgroup a CU of its own; never more than run
+    // co-resident (the members of a replica wait for each other)
+    const int R = (int)c->reps.size();
+    int coop = c->mc_coop > 0 ? c->mc_coop : REMC_MAX_COOP;
+    if (c->mc_coop <= 0)
+        while (coop > 1 && (long long)R * coop > (long long)c->n_cu) coop >>= 1;
+    if (coop > 1) {
+        int per_cu = 0;
+        HIP_TRY(mc_fb_max_active_per_cu(p, block, &per_cu));
+        while (coop > 1 && (long long)R * coop > (long long)per_cu * c->n_cu) coop >>= 1;
+    }
+    *out = p;
+    *coop_out = coop;
+    return REMC_OK;
+}
+
+// Workgroup size of the force-biased kernel: each of 256, 512 and 1024
+// threads runs 128 steps on the live state (best of two), which is then
+// restored bit for bit.
+int autotune_fb(remc_ctx *c)
+{
+    StateGuard st(c);
+    int rc = st.save();
+    if (rc != REMC_OK) return rc;
+    EventPair ev;
+    HIP_TRY(ev.create());
+    const int R = (int)c->reps.size();
+    const int blocks[3] = {256, 512, 1024};
+    float best = 1e30f;
+    int best_block = 0;
+    for (int bi = 0; bi < 3; ++bi) {
+        KParams p{};
+        int coop = 1;
+        rc = fb_prepare(c, blocks[bi], &p, &coop);
+        if (rc == REMC_ERR_LIMIT) continue;
+        if (rc != REMC_OK) return rc;
+        p.rec = nullptr;
+        float ms_best = 1e30f;
+        for (int rep = 0; rep < 2; ++rep) {
+            rc = sync_copies(c, coop);
+            if (rc != REMC_OK) return rc;
+            HIP_TRY(launch_mc_fb(p, R, 4, blocks[bi], coop, c->stream)); // warm
+            HIP_TRY(hipEventRecord(ev.a, c->stream));
+            HIP_TRY(launch_mc_fb(p, R, 128, blocks[bi], coop, c->stream));
+            HIP_TRY(hipEventRecord(ev.b, c->stream));
+            HIP_TRY(hipEventSynchronize(ev.b));
+            rc = check_err(c);
+            if (rc != REMC_OK) return rc;
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            ms_best = std::min(ms_best, ms);
+            rc = st.restore();
+            if (rc != REMC_OK) return rc;
+        }
+        if (ms_best < best) {
+            best = ms_best;
+            best_block = blocks[bi];
+        }
+    }
+    if (!best_block) return fail(REMC_ERR_LIMIT, "no workgroup size of the force-biased kernel fits this context");
+    c->fb_block = best_block;
     return REMC_OK;
 }
 
@@ -974,7 +1145,7 @@ int remc_energy_all(remc_ctx *c, double *out)
     const int R = (int)c->reps.size();
     {
         TimedLaunch t(c, 1);
-        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, c->stream));
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 0, 0, c->full_split, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(out, c->d_out, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -998,7 +1169,7 @@ int remc_energy_all_noalloc(remc_ctx *c, double *out)
         return fail(REMC_ERR_LIMIT, "remc_energy_all_noalloc needs %zu doubles of scratch for %d replicas of up to %d "
                                     "beads; its static scratch holds %zu", need, R, c->max_beads, REMC_NOALLOC_DOUBLES);
     double *partials = c->noalloc_scratch, *dout = partials + stride * (size_t)R;
-    HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, partials, dout, 0, 0, c->stream));
+    HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, partials, dout, 0, 0, c->full_split, c->stream));
     HIP_TRY(hipMemcpyAsync(out, dout, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
@@ -1011,7 +1182,7 @@ int remc_refresh_energies(remc_ctx *c)
     const int R = (int)c->reps.size();
     {
         TimedLaunch t(c, 1);
-        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, c->stream));
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, c->full_split, c->stream));
     }
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
@@ -1025,7 +1196,7 @@ int remc_energy_noncrowder_all(remc_ctx *c, double *out)
     const int R = (int)c->reps.size();
     {
         TimedLaunch t(c, 1);
-        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 1, 0, c->stream));
+        HIP_TRY(launch_energy_full(c->params(), R, c->max_tiles, c->d_full_partials, c->d_out, 1, 0, c->full_split, c->stream));
     }
     HIP_TRY(hipMemcpyAsync(out, c->d_out, sizeof(double) * R, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1054,45 +1225,27 @@ int remc_mc_steps_fb(remc_ctx *c, int nsteps, int block)
     CHECK_CTX(c);
     FINALIZE(c);
     if (nsteps < 0) return fail(REMC_ERR_ARG, "negative step count");
-    if (block == 0) block = 512;
-    if (block != 256 && block != 512 && block != 1024)
-        return fail(REMC_ERR_ARG, "workgroup size %d (256, 512 or 1024)", block);
+    if (block != 0 && block != 256 && block != 512 && block != 1024)
+        return fail(REMC_ERR_ARG, "workgroup size %d (256, 512 or 1024; 0 = autotuned)", block);
     if (nsteps == 0) return REMC_OK;
-    KParams p = c->params();
-    p.lds_mb = std::max(1, c->max_body_beads);
-    p.lds_cl = 0;
-    const long long static_lds = 13 * 1024; // table, reductions, exchange, 64 batched draws
-    if ((long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block)
-        return fail(REMC_ERR_LIMIT, "a body of %d beads does not fit the force-biased kernel's LDS staging",
-                    c->max_body_beads);
-    // the replica's partner beads with their relative positions, and its body
-    // centres and rotations, in LDS when they all fit (else read from HBM)
-    p.lds_cl = c->max_beads;
-    p.lds_nbody = c->max_bodies;
-    if (!c->lds_cache || (long long)mc_fb_dynamic_lds(p) + static_lds > c->lds_per_block) {
-        p.lds_cl = 0;
-        p.lds_nbody = 0;
+    int rc = REMC_OK;
+    if (block == 0) {
+        if (c->fb_block == 0) {
+            rc = autotune_fb(c);
+            if (rc != REMC_OK) return rc;
+        }
+        block = c->fb_block;
     }
-    // windows of one-bead crowder moves (with the cache), switched with the MC kernel's
-    p.spec = c->mc_window ? 1 : 0;
-    // workgroups per replica: the MC kernel's setting when one is made, else
-    // as many as give every workgroup a CU of its own; never more than run
-    // co-resident (the members of a replica wait for each other)
-    const int R = (int)c->reps.size();
-    int coop = c->mc_coop > 0 ? c->mc_coop : REMC_MAX_COOP;
-    if (c->mc_coop <= 0)
-        while (coop > 1 && (long long)R * coop > (long long)c->n_cu) coop >>= 1;
-    if (coop > 1) {
-        int per_cu = 0;
-        HIP_TRY(mc_fb_max_active_per_cu(p, block, &per_cu));
-        while (coop > 1 && (long long)R * coop > (long long)per_cu * c->n_cu) coop >>= 1;
-    }
+    KParams p{};
+    int coop = 1;
+    rc = fb_prepare(c, block, &p, &coop);
+    if (rc != REMC_OK) return rc;
     // every member starts from the same state
-    int rc = sync_copies(c, coop);
+    rc = sync_copies(c, coop);
     if (rc != REMC_OK) return rc;
     {
         TimedLaunch t(c, 0);
-        HIP_TRY(launch_mc_fb(p, R, nsteps, block, coop, c->stream));
+        HIP_TRY(launch_mc_fb(p, (int)c->reps.size(), nsteps, block, coop, c->stream));
     }
     c->synced_copies = coop;
     return REMC_OK;
@@ -1241,6 +1394,10 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
         hipGraph_t g = nullptr;
         HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         hipError_t e1 = launch_mc(p, (int)c->reps.size(), steps_per_round, c->mc_block, c->mc_coop, c->stream);
+        // with the refresh, the exchange reads full energies, as the reference's does
+        if (e1 == hipSuccess && c->run_refresh)
+            e1 = launch_energy_full(p, (int)c->reps.size(), c->max_tiles, c->d_full_partials, nullptr, 0, 1,
+                                    c->full_split, c->stream);
         hipError_t e2 = launch_exchange(p, (int)c->systems.size(), c->stream);
         hipError_t e3 = hipStreamEndCapture(c->stream, &g);
         if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
@@ -1259,6 +1416,15 @@ int remc_run(remc_ctx *c, int rounds, int steps_per_round)
         TimedLaunch t(c, 0);
         HIP_TRY(hipGraphLaunch(c->graph, c->stream));
     }
+    return REMC_OK;
+}
+
+int remc_set_run_refresh(remc_ctx *c, int enable)
+{
+    CHECK_CTX(c);
+    const bool on = enable != 0;
+    if (on != c->run_refresh) destroy_graph(c); // the captured rounds change
+    c->run_refresh = on;
     return REMC_OK;
 }
 
@@ -1674,6 +1840,14 @@ int remc_set_mc_window(remc_ctx *c, int enable)
     CHECK_CTX(c);
     c->mc_window = enable != 0;
     destroy_graph(c);
+    return REMC_OK;
+}
+
+int remc_get_fb_block(remc_ctx *c, int *block)
+{
+    CHECK_CTX(c);
+    if (!block) return fail(REMC_ERR_ARG, "null output");
+    *block = c->fb_block;
     return REMC_OK;
 }
 

@@ -107,8 +107,11 @@ hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, int 
 hipError_t mc_fb_max_active_per_cu(const KParams &p, int block, int *out);
 size_t mc_fb_dynamic_lds(const KParams &p);
 hipError_t mc_max_active_per_cu(const KParams &p, int block, int *out);
+// split: partner sub-ranges per full-energy unit (1, 2 or FULL_SPLIT_MAX;
+// anything else takes full_split_default, a rule on the grid's size)
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
-                              int noncrowder_only, int write_E, hipStream_t s);
+                              int noncrowder_only, int write_E, int split, hipStream_t s);
+int full_split_default(int n_rep, int max_tiles);
 hipError_t launch_energy_plain(const KParams &p, int replica, int n_beads, int block, float *partials, hipStream_t s);
 hipError_t launch_exchange(const KParams &p, int n_sys, hipStream_t s);
 // adjoint launch plan for n_rep replicas of at most max_beads beads whose

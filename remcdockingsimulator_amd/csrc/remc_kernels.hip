@@ -2155,13 +2155,18 @@ __global__ __launch_bounds__(64) void k_sum_partials(KParams p, int stride, int 
     }
 }
 
+int full_split_default(int n_rep, int max_tiles)
+{
+    const long long units = (long long)full_units(max_tiles) * n_rep;
+    return units >= 2048 ? 1 : units >= 1024 ? 2 : FULL_SPLIT_MAX;
+}
+
 hipError_t launch_energy_full(const KParams &p, int n_rep, int max_tiles, double *partials, double *out,
-                              int noncrowder_only, int write_E, hipStream_t s)
+                              int noncrowder_only, int write_E, int split, hipStream_t s)
 {
     if (n_rep <= 0) return hipSuccess;
     const int max_units = full_units(max_tiles);
-    const long long units = (long long)max_units * n_rep;
-    const int split = units >= 2048 ? 1 : units >= 1024 ? 2 : FULL_SPLIT_MAX;
+    if (split != 1 && split != 2 && split != FULL_SPLIT_MAX) split = full_split_default(n_rep, max_tiles);
     const int stride = max_units * FULL_SPLIT_MAX;
     hipLaunchKernelGGL(k_energy_full, dim3(max_units * split, n_rep), dim3(FULL_TILE), 0, s, p, stride, split,
                        partials, noncrowder_only);

@@ -142,6 +142,10 @@ class Engine:
     def run(self, rounds, steps_per_round):
         self._chk(self.lib.remc_run(self.ctx, int(rounds), int(steps_per_round)), "remc_run")
 
+    def set_run_refresh(self, enable=True):
+        """Energy refresh inside run()'s captured rounds, before each exchange."""
+        self._chk(self.lib.remc_set_run_refresh(self.ctx, 1 if enable else 0), "remc_set_run_refresh")
+
     def synchronize(self):
         self._chk(self.lib.remc_synchronize(self.ctx), "remc_synchronize")
 
@@ -162,6 +166,11 @@ class Engine:
     def mc_block(self):
         b = ctypes.c_int()
         self._chk(self.lib.remc_get_mc_block(self.ctx, ctypes.byref(b)), "remc_get_mc_block")
+        return b.value
+
+    def fb_block(self):
+        b = ctypes.c_int()
+        self._chk(self.lib.remc_get_fb_block(self.ctx, ctypes.byref(b)), "remc_get_fb_block")
         return b.value
 
     def set_mc_coop(self, coop):

@@ -172,6 +172,30 @@ def test_fb_uneven_systems_in_one_launch(oracle):
     assert diverged <= 1
 
 
+def test_fb_autotuned_block_keeps_state_and_matches_explicit_block(oracle):
+    """block = 0: the autotuner times each workgroup size on the live state and
+    restores it bit for bit, so the steps then taken equal those of a fresh
+    context run with the chosen size."""
+    wl = synth.config(4)
+    a = engine_for(wl, seed=77)
+    a.set_mc_coop(8)
+    a.mc_steps_fb(300, 0)
+    a.synchronize()
+    chosen = a.fb_block()
+    say("force-biased autotuner picked", chosen)
+    assert chosen in (256, 512, 1024)
+    b = engine_for(wl, seed=77)
+    b.set_mc_coop(8)
+    b.mc_steps_fb(300, chosen)
+    b.synchronize()
+    assert np.array_equal(a.counters(), b.counters())
+    assert np.array_equal(a.energies(), b.energies())
+    for r in (0, 31):
+        assert np.array_equal(a.state(r)["pos"], b.state(r)["pos"])
+    a.close()
+    b.close()
+
+
 def test_fb_rejects_bad_block():
     wl = synth.config(1)
     e = engine_for(wl)

@@ -352,6 +352,54 @@ def test_graph_run_equals_eager(oracle):
     b.close()
 
 
+def test_graph_run_with_refresh_equals_eager(oracle):
+    """remc_run with the energy refresh before every exchange
+    (remc_set_run_refresh) equals the eager sequence MC steps, refresh,
+    exchange bit for bit, and leaves the tracked energies equal to the full
+    evaluation of the conformations."""
+    wl = synth.config(4)
+    a = engine_for(wl, seed=SEED, block=1024)
+    b = engine_for(wl, seed=SEED, block=1024)
+    a.set_mc_coop(8)
+    b.set_mc_coop(8)
+    a.set_run_refresh(True)
+    for k in range(2):
+        a.run(3, 300)
+        for _ in range(3):
+            b.mc_steps(300)
+            b.refresh_energies()
+            b.exchange()
+        a.synchronize()
+        b.synchronize()
+        say("refreshing graph replay batch", k)
+        assert np.array_equal(a.energies(), b.energies())
+        assert np.array_equal(a.temperatures(), b.temperatures())
+        assert np.array_equal(a.ladder(0), b.ladder(0))
+        for r in (0, 13, 31):
+            assert np.array_equal(a.state(r)["pos"], b.state(r)["pos"])
+    assert np.array_equal(a.energies(), a.energy_all())  # the last round's exchange read refreshed energies
+    a.set_run_refresh(False)  # recaptured without the refresh: back to the plain rounds
+    a.run(1, 300)
+    a.synchronize()
+    a.close()
+    b.close()
+
+
+def test_energy_split_agrees_across_contexts(oracle):
+    """The full-energy split is timed once per process and grid size: two
+    contexts of one size evaluate the same bits, and both match the oracle."""
+    wl = synth.config(2)
+    a = engine_for(wl, seed=SEED)
+    b = engine_for(wl, seed=SEED + 1)
+    ea, eb = a.energy_all(), b.energy_all()
+    assert np.array_equal(ea, eb)  # same starting conformations
+    osys = oracle.OracleSystem(wl.systems[0], wl.ff)
+    want = osys.energy(a.state(0)["pos"])
+    assert abs(ea[0] - want) <= 1e-5 * abs(want)
+    a.close()
+    b.close()
+
+
 def test_single_body_and_all_crowder_systems(oracle):
     """Edge systems: one body (dE = 0, every move accepted) and crowders only."""
     from remcdockingsimulator_amd.engine import Engine
