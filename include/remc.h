@@ -247,6 +247,18 @@ REMC_API int remc_drms(remc_ctx *ctx, int sys_id, const float *ref_x, const floa
 REMC_API int remc_centre_histogram(remc_ctx *ctx, int sys_id, const int *body_type, int n_types, int nbin, double lo,
                                    double hi, uint64_t *counts);
 
+/* Radial distribution g(r) of body pairs by type over every replica of
+ * system sys_id, each replica one frame, as "post processor/gr.py:93-155"
+ * computes it from PDB frames: the histogram of remc_centre_histogram (same
+ * type and bin conventions; a body's point is the position of its bead
+ * site_bead[m], which gr.py takes as the residue nearest the chain's centroid,
+ * or its centre when site_bead is NULL), divided by
+ * n_frames 4 pi r^2 dr n_ti n_tj / V with r the bin centre, n_t the number of
+ * bodies of type t and V = box^3 (the system must be periodic).
+ * gr[(ti * n_types + tj) * nbin + bin] for ti <= tj (0 below the diagonal). */
+REMC_API int remc_radial_distribution(remc_ctx *ctx, int sys_id, const int *body_type, const int *site_bead,
+                                      int n_types, int nbin, double lo, double hi, double *gr);
+
 /* ------------------------------------------------------------- adjoint --- */
 /* Per-body force and torque about the body centre (6 doubles per body:
  * Fx Fy Fz Tx Ty Tz, kcal/mol/A and kcal/mol/rad, F = -dE/dx) and the

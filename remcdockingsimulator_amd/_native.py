@@ -84,6 +84,8 @@ SIGNATURES = {
                                                ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)]),
     "remc_set_exchange_state": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_uint64, ctypes.c_int, ctypes.c_uint64]),
     "remc_drms": (ctypes.c_int, [_vp, ctypes.c_int, _fp, _fp, _fp, _dp]),
+    "remc_radial_distribution": (ctypes.c_int, [_vp, ctypes.c_int, _ip, _ip, ctypes.c_int, ctypes.c_int,
+                                                 ctypes.c_double, ctypes.c_double, _dp]),
     "remc_centre_histogram": (ctypes.c_int, [_vp, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]),
     "remc_adjoint": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp]),

@@ -115,9 +115,11 @@ constexpr int HIST_BLOCK = 256;
 
 // counts[(ti * n_types + tj) * nbin + bin] for ti <= tj; a pair of bodies of
 // the same type counts once from each side, a pair of different types once
-// (gr.py's loops over typeI <= typeJ and over both chains).
-__global__ __launch_bounds__(HIST_BLOCK) void k_centre_hist(KParams p, int sys_id, const int *btype, int n_types,
-                                                            int nbin, double lo, double dr,
+// (gr.py's loops over typeI <= typeJ and over both chains).  A body's point is
+// its centre, or with `site` the position of its bead site[m] (gr.py takes the
+// residue nearest each chain's centroid).
+__global__ __launch_bounds__(HIST_BLOCK) void k_centre_hist(KParams p, int sys_id, const int *btype, const int *site,
+                                                            int n_types, int nbin, double lo, double dr,
                                                             unsigned long long *counts)
 {
     extern __shared__ unsigned int s_hist[];
@@ -130,14 +132,16 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_centre_hist(KParams p, int sys_i
     __syncthreads();
     const int a = blockIdx.x * HIST_BLOCK + threadIdx.x;
     const float *cx = p.cx + R.body_off, *cy = p.cy + R.body_off, *cz = p.cz + R.body_off;
+    const float *bx = p.px + R.bead_off, *by = p.py + R.bead_off, *bz = p.pz + R.bead_off;
     const double L = (double)S.box;
     if (a < M && btype[a] >= 0) {
         const int ta = btype[a];
-        const double xa = cx[a], ya = cy[a], za = cz[a];
+        const double xa = site ? bx[site[a]] : cx[a], ya = site ? by[site[a]] : cy[a], za = site ? bz[site[a]] : cz[a];
         for (int b = 0; b < M; ++b) {
             const int tb = btype[b];
             if (b == a || tb < 0 || tb < ta) continue;
-            double dx = __dsub_rn(xa, (double)cx[b]), dy = __dsub_rn(ya, (double)cy[b]), dz = __dsub_rn(za, (double)cz[b]);
+            const double xb = site ? bx[site[b]] : cx[b], yb = site ? by[site[b]] : cy[b], zb = site ? bz[site[b]] : cz[b];
+            double dx = __dsub_rn(xa, xb), dy = __dsub_rn(ya, yb), dz = __dsub_rn(za, zb);
             if (L > 0.0) {
                 dx = __dsub_rn(dx, __dmul_rn(L, round(__ddiv_rn(dx, L))));
                 dy = __dsub_rn(dy, __dmul_rn(L, round(__ddiv_rn(dy, L))));
@@ -154,13 +158,39 @@ __global__ __launch_bounds__(HIST_BLOCK) void k_centre_hist(KParams p, int sys_i
         if (s_hist[t]) atomicAdd(&counts[t], (unsigned long long)s_hist[t]);
 }
 
-hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, int n_types,
-                              int nbin, double lo, double dr, unsigned long long *counts, hipStream_t s)
+hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, const int *site,
+                              int n_types, int nbin, double lo, double dr, unsigned long long *counts, hipStream_t s)
 {
     if (n_rep <= 0 || n_bodies <= 0) return hipSuccess;
     const size_t lds = sizeof(unsigned int) * (size_t)n_types * n_types * nbin;
     hipLaunchKernelGGL(k_centre_hist, dim3((n_bodies + HIST_BLOCK - 1) / HIST_BLOCK, n_rep), dim3(HIST_BLOCK), lds, s,
-                       p, sys_id, btype, n_types, nbin, lo, dr, counts);
+                       p, sys_id, btype, site, n_types, nbin, lo, dr, counts);
+    return hipGetLastError();
+}
+
+// g = counts / (frames 4 pi r^2 dr n_ti n_tj / V), r the bin centre (gr.py:140-155)
+__global__ void k_gr_normalise(const unsigned long long *counts, const int *type_count, int n_types, int nbin,
+                               double lo, double dr, double frames_over_volume, double *gr)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_types * n_types * nbin) return;
+    const int b = t % nbin, pair = t / nbin, ti = pair / n_types, tj = pair % n_types;
+    if (ti > tj) {
+        gr[t] = 0.0;
+        return;
+    }
+    const double r = lo + (0.5 + (double)b) * dr;
+    const double four_pi = 12.566370614359172953850573533118;
+    const double norm = frames_over_volume * four_pi * r * r * dr * (double)type_count[ti] * (double)type_count[tj];
+    gr[t] = norm > 0.0 ? (double)counts[t] / norm : 0.0;
+}
+
+hipError_t launch_gr_normalise(const unsigned long long *counts, const int *type_count, int n_types, int nbin, double lo,
+                               double dr, double frames_over_volume, double *gr, hipStream_t s)
+{
+    const int n = n_types * n_types * nbin;
+    hipLaunchKernelGGL(k_gr_normalise, dim3((n + 255) / 256), dim3(256), 0, s, counts, type_count, n_types, nbin, lo, dr,
+                       frames_over_volume, gr);
     return hipGetLastError();
 }
 

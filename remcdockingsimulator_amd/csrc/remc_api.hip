@@ -1738,9 +1738,54 @@ int remc_centre_histogram(remc_ctx *c, int sys_id, const int *body_type, int n_t
     unsigned long long *dcounts = (unsigned long long *)((char *)c->d_ana + tbytes);
     HIP_TRY(hipMemcpyAsync(dtype, body_type, sizeof(int) * M, hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemsetAsync(dcounts, 0, sizeof(unsigned long long) * nh, c->stream));
-    HIP_TRY(launch_centre_hist(c->params(), sys_id, M, R, dtype, n_types, nbin, lo, (hi - lo) / nbin, dcounts,
+    HIP_TRY(launch_centre_hist(c->params(), sys_id, M, R, dtype, nullptr, n_types, nbin, lo, (hi - lo) / nbin, dcounts,
                                c->stream));
     HIP_TRY(hipMemcpyAsync(counts, dcounts, sizeof(unsigned long long) * nh, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return REMC_OK;
+}
+
+int remc_radial_distribution(remc_ctx *c, int sys_id, const int *body_type, const int *site_bead, int n_types,
+                             int nbin, double lo, double hi, double *gr)
+{
+    CHECK_CTX(c);
+    FINALIZE(c);
+    if (sys_id < 0 || sys_id >= (int)c->systems.size()) return fail(REMC_ERR_ARG, "bad system %d", sys_id);
+    if (!body_type || !gr) return fail(REMC_ERR_ARG, "null argument");
+    if (n_types < 1 || nbin < 1 || !(hi > lo) || !std::isfinite(lo) || !std::isfinite(hi))
+        return fail(REMC_ERR_ARG, "need n_types >= 1, nbin >= 1 and lo < hi");
+    const HostSys &s = c->systems[sys_id];
+    if (!(s.box > 0.f)) return fail(REMC_ERR_ARG, "g(r) needs a periodic box (the volume normalises it)");
+    const size_t nh = (size_t)n_types * n_types * nbin;
+    if (nh * sizeof(unsigned int) > 48 * 1024)
+        return fail(REMC_ERR_LIMIT, "histogram of %zu bins exceeds the 12288-bin workgroup limit", nh);
+    const int M = s.n_bodies, R = (int)s.replicas.size();
+    std::vector<int> type_count(n_types, 0);
+    for (int m = 0; m < M; ++m) {
+        if (body_type[m] < -1 || body_type[m] >= n_types) return fail(REMC_ERR_ARG, "body %d has type %d", m, body_type[m]);
+        if (body_type[m] >= 0) ++type_count[body_type[m]];
+        if (site_bead && body_type[m] >= 0 &&
+            (site_bead[m] < s.body_start[m] || site_bead[m] >= s.body_start[m + 1]))
+            return fail(REMC_ERR_ARG, "site bead %d is not a bead of body %d", site_bead[m], m);
+    }
+    auto up = [](size_t b) { return ((b + 255) / 256) * 256; };
+    const size_t tb = up(sizeof(int) * (size_t)M), sb = up(sizeof(int) * (size_t)M), cb = up(sizeof(int) * n_types);
+    const size_t hb = up(sizeof(unsigned long long) * nh);
+    int rc = ana_reserve(c, tb + sb + cb + hb + sizeof(double) * nh);
+    if (rc != REMC_OK) return rc;
+    char *base = (char *)c->d_ana;
+    int *dtype = (int *)base, *dsite = (int *)(base + tb), *dcount = (int *)(base + tb + sb);
+    unsigned long long *dhist = (unsigned long long *)(base + tb + sb + cb);
+    double *dgr = (double *)(base + tb + sb + cb + hb);
+    HIP_TRY(hipMemcpyAsync(dtype, body_type, sizeof(int) * M, hipMemcpyHostToDevice, c->stream));
+    if (site_bead) HIP_TRY(hipMemcpyAsync(dsite, site_bead, sizeof(int) * M, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemcpyAsync(dcount, type_count.data(), sizeof(int) * n_types, hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipMemsetAsync(dhist, 0, sizeof(unsigned long long) * nh, c->stream));
+    const double dr = (hi - lo) / nbin, L = (double)s.box;
+    HIP_TRY(launch_centre_hist(c->params(), sys_id, M, R, dtype, site_bead ? dsite : nullptr, n_types, nbin, lo, dr,
+                               dhist, c->stream));
+    HIP_TRY(launch_gr_normalise(dhist, dcount, n_types, nbin, lo, dr, (double)R / (L * L * L), dgr, c->stream));
+    HIP_TRY(hipMemcpyAsync(gr, dgr, sizeof(double) * nh, hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     return REMC_OK;
 }

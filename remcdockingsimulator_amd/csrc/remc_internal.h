@@ -128,8 +128,10 @@ hipError_t launch_adjoint(const KParams &p, const void *adj_ff, const AdjPlan &a
                           double *force_torque, double *dparams, hipStream_t s);
 hipError_t launch_drms(const KParams &p, int sys_id, int n_beads, int n_rep, const float *rx, const float *ry,
                        const float *rz, double *partials, double inv_pairs, double *out, hipStream_t s);
-hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, int n_types,
-                              int nbin, double lo, double dr, unsigned long long *counts, hipStream_t s);
+hipError_t launch_centre_hist(const KParams &p, int sys_id, int n_bodies, int n_rep, const int *btype, const int *site,
+                              int n_types, int nbin, double lo, double dr, unsigned long long *counts, hipStream_t s);
+hipError_t launch_gr_normalise(const unsigned long long *counts, const int *type_count, int n_types, int nbin, double lo,
+                               double dr, double frames_over_volume, double *gr, hipStream_t s);
 size_t adjoint_ff_size();
 void adjoint_ff_fill(void *dst, const float *table, double lambda, double e0, double xi, double c_lj, double c_dh);
 

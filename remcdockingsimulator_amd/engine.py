@@ -285,6 +285,17 @@ class Engine:
         self._chk(self.lib.remc_drms(self.ctx, sys_id, *(nat.fptr(c) for c in cols), nat.dptr(out)), "remc_drms")
         return out
 
+    def radial_distribution(self, sys_id, body_type, n_types, nbin, lo, hi, site_bead=None):
+        """g(r) by body-type pair over every replica of a system (gr.py's
+        normalisation): array (n_types, n_types, nbin), zero below the diagonal."""
+        bt = np.ascontiguousarray(body_type, dtype=np.int32)
+        site = None if site_bead is None else np.ascontiguousarray(site_bead, dtype=np.int32)
+        out = np.zeros(n_types * n_types * nbin, dtype=np.float64)
+        self._chk(self.lib.remc_radial_distribution(self.ctx, int(sys_id), nat.iptr(bt),
+                                                    None if site is None else nat.iptr(site), int(n_types), int(nbin),
+                                                    float(lo), float(hi), nat.dptr(out)), "remc_radial_distribution")
+        return out.reshape(n_types, n_types, nbin)
+
     def centre_histogram(self, sys_id, body_type, n_types, nbin, lo, hi):
         """Counts (n_types, n_types, nbin) of body-centre distances over the system's replicas."""
         bt = np.ascontiguousarray(body_type, dtype=np.int32)

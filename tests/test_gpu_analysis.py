@@ -1,9 +1,11 @@
 """Device analysis reductions through the C ABI against numpy restatements
 of the reference's post-processing: DRMS ("post processor/pdbgensrc/
 drms.cpp:7-58") and the centroid g(r) histogram ("post processor/gr.py:
-93-140").  Bars: DRMS within 1e-6 relative (FP32 distances, the same
-operation sequence); histogram counts exact (FP64 distances with the same
-operation sequence as the numpy restatement)."""
+93-140"), and the normalised g(r) ("gr.py:140-155").  Bars: DRMS within 1e-6
+relative (FP32 distances, the same operation sequence); histogram counts
+exact (FP64 distances with the same operation sequence as the numpy
+restatement); g(r) of uncorrelated points at its closed form, 1 between
+types and (n - 1)/n within one type, within statistical error."""
 import numpy as np
 import pytest
 
@@ -96,3 +98,111 @@ def test_centre_histogram_against_numpy():
     with pytest.raises(RemcError):
         e.centre_histogram(0, btype, n_types, 5000, lo, hi)  # beyond the bin limit
     e.close()
+
+
+def random_frames_system(n_chains, n_frames, box, seed, beads_per_chain=3):
+    """n_chains of beads_per_chain beads; each frame (replica) places the chains
+    uniformly and independently in the box."""
+    from remcdockingsimulator_amd import forcefield as ffm
+    from remcdockingsimulator_amd.engine import Engine
+    from remcdockingsimulator_amd.model import SystemArrays
+
+    rng = np.random.default_rng(seed)
+    n = n_chains * beads_per_chain
+    start = np.arange(0, n + 1, beads_per_chain, dtype=np.int32)
+    rel = np.tile(np.array([[-3.8, 0, 0], [0, 0, 0], [3.8, 0, 0]], np.float32)[:beads_per_chain], (n_chains, 1))
+    cen = (rng.random((n_chains, 3)) * box).astype(np.float32)
+    body = np.repeat(np.arange(n_chains), beads_per_chain)
+    sa = SystemArrays(body_start=start, type=np.zeros(n, np.int32), charge=np.zeros(n, np.float32),
+                      sigma=np.full(n, 6.0, np.float32), crowder=np.zeros(n_chains, np.uint8), box=float(np.float32(box)),
+                      pos=(cen[body] + rel).astype(np.float32), rel=rel, center=cen,
+                      quat=np.tile([1.0, 0, 0, 0], (n_chains, 1)))
+    e = Engine(0)
+    e.set_forcefield(ffm.ForceField.synthetic(1))
+    e.add_system(sa)
+    e.add_replicas(0, [300.0 + 1e-3 * k for k in range(n_frames)], 2.0, 0.2)
+    frames = []
+    for k in range(n_frames):
+        c = (rng.random((n_chains, 3)) * box).astype(np.float32)
+        p = (c[body] + rel).astype(np.float32)
+        e.set_state(k, pos=p)
+        frames.append(p)
+    return e, sa, frames
+
+
+def test_radial_distribution_of_uncorrelated_points():
+    """Chains placed independently and uniformly: gr.py's normalisation
+    (n_frames 4 pi r^2 dr n_I n_J / V over ordered pairs of distinct chains)
+    gives g = 1 between two types and (n_I - 1)/n_I within one type; a
+    normalisation by n_I (n_I - 1) or without the frame count would miss both."""
+    nA, nB, frames, box = 6, 4, 3000, 100.0
+    e, sa, _ = random_frames_system(nA + nB, frames, box, seed=3)
+    btype = np.array([0] * nA + [1] * nB, np.int32)
+    site = np.arange(nA + nB, dtype=np.int32) * 3 + 1  # the middle bead
+    g = e.radial_distribution(0, btype, 2, 75, 0.0, box / 2 - 1.0, site_bead=site)
+    mid = slice(10, 75)  # away from the sparse first bins
+    say("g(r) means: AA %.4f BB %.4f AB %.4f" % (g[0, 0, mid].mean(), g[1, 1, mid].mean(), g[0, 1, mid].mean()))
+    assert abs(g[0, 1, mid].mean() - 1.0) < 0.03
+    assert abs(g[0, 0, mid].mean() - (nA - 1) / nA) < 0.03
+    assert abs(g[1, 1, mid].mean() - (nB - 1) / nB) < 0.04
+    assert np.all(g[1, 0] == 0.0)
+    e.close()
+
+
+def test_radial_distribution_counts_site_beads_exactly():
+    """g times its normaliser is the exact histogram of the site beads'
+    minimum-image distances (numpy restatement of gr.py's loops)."""
+    nA, nB, frames, box = 5, 3, 40, 60.0
+    e, sa, pos = random_frames_system(nA + nB, frames, box, seed=8)
+    btype = np.array([0] * nA + [1] * nB, np.int32)
+    site = np.arange(nA + nB, dtype=np.int32) * 3 + np.arange(nA + nB) % 3
+    nbin, lo, hi = 75, 0.0, box / 2 - 1.0
+    g = e.radial_distribution(0, btype, 2, nbin, lo, hi, site_bead=site)
+    want = np.zeros((2, 2, nbin), np.uint64)
+    for p in pos:
+        want += hist_numpy(p[site], btype, 2, nbin, lo, hi, float(np.float32(box)))
+    dr = (hi - lo) / nbin
+    r = lo + (0.5 + np.arange(nbin)) * dr
+    L = float(np.float32(box))
+    n = np.array([nA, nB], float)
+    for i in range(2):
+        for j in range(i, 2):
+            norm = frames * 4 * np.pi * r * r * dr * n[i] * n[j] / L ** 3
+            assert np.allclose(g[i, j] * norm, want[i, j].astype(float), rtol=1e-12, atol=1e-9)
+    with pytest.raises(RemcError):
+        e.radial_distribution(0, btype, 2, nbin, lo, hi, site_bead=site + 3)  # beads of other chains
+    e.close()
+
+
+def test_radial_tool_writes_gr_py_files(tmp_path):
+    """remcdockingsimulator_amd.radial on PDB frames in the reference's writer
+    format: one gr_<I>_<J>.dat per type pair, "%8.3f %8.3f" lines of the bin
+    centre and g."""
+    from remcdockingsimulator_amd import radial
+
+    rng = np.random.default_rng(5)
+    box = 80.0
+    names = ["ALA", "GLY", "LYS"]
+    paths = []
+    for k in range(30):
+        chains = []
+        for c, cid in enumerate("ABCDE"):
+            seq = names if cid in "ABC" else ["TRP", "TRP"]
+            x0 = rng.random(3) * box
+            chains.append((cid, [(nm, i + 1, tuple(x0 + [3.8 * i, 0, 0])) for i, nm in enumerate(seq)]))
+        p = str(tmp_path / ("frame%03d.pdb" % k))
+        with open(p, "w") as f:
+            n = 0
+            for cid, res in chains:
+                for nm, num, (x, y, z) in res:
+                    n += 1
+                    f.write("ATOM  %5d %4s%c%3s %c%4d%c  %8.3f%8.3f%8.3f%6.2f%6.2f\n" % (n, "CA", " ", nm, cid, num, " ",
+                                                                                        x, y, z, 1.0, 1.0))
+        paths.append(p)
+    radial.main([str(box)] + paths + ["--outdir", str(tmp_path)])
+    keys, r, g = radial.radial_distribution(box, paths)
+    assert keys == ["A", "D"]
+    for i, j in ((0, 0), (0, 1), (1, 1)):
+        lines = open(str(tmp_path / ("gr_%s_%s.dat" % (keys[i], keys[j])))).read().splitlines()
+        assert len(lines) == 75
+        assert lines[3] == "%8.3f %8.3f" % (r[3], g[i, j, 3])
