@@ -504,6 +504,51 @@ double Replica::E(int device)
     return e;
 }
 
+int checkConformations(const AminoAcids &aas, const argdata &p)
+{
+    // main.cpp:1652-1707 sums each example by Replica::E() on the host and by
+    // EonDevice; here the second device path, the batched full-energy kernel
+    // (remc_energy_all), stands where the host sum stood.
+    std::printf("performing check...\n");
+    const float box = 118.4f;
+    for (int i = 1; i <= 10; ++i) {
+        Replica r;
+        r.setAminoAcidData(aas);
+        r.setLabel(i);
+        r.setBoundingValue(box);
+        char a[4096], b[4096];
+        std::snprintf(a, sizeof a, "%s/conf%d/1a.pdb", p.checkDir.c_str(), i);
+        std::snprintf(b, sizeof b, "%s/conf%d/1b.pdb", p.checkDir.c_str(), i);
+        if (r.loadMolecule(a) < 0 || r.loadMolecule(b) < 0) {
+            std::printf("! cannot load example conformation %d (%s, %s)\n", i, a, b);
+            return 1;
+        }
+        r.nonCrowderCount = (int)r.molecules.size();
+        const double batched = r.E(p.device);
+        const double plain = r.EonDevice();
+        r.FreeDevice();
+        if (batched != batched || plain != plain) return 1;
+        std::printf("PLAIN,BATCHED,err: %24.20f %24.20f %24.20f\n", plain, batched, std::fabs(plain - batched));
+    }
+    std::fflush(stdout);
+    return 0;
+}
+
+double checkedInitialEnergy(Replica *initial, int device)
+{
+    const double e = initial->E(device);
+    if (e != e) return e;
+    const double plain = initial->EonDevice();
+    initial->FreeDevice();
+    if (plain != plain) return plain;
+    std::printf("GPU initial energy value: %26.23f kcal/mol\n", e);
+    std::printf("Absolute error:           %26.23f kcal/mol\n", std::fabs(e - plain));
+    std::printf("Relative error:           %26.23f kcal/mol\n", std::fabs(plain - e) / std::fabs(plain));
+    if (std::fabs(plain - e) / std::fabs(plain) > 0.01)
+        std::printf("!\n! WARNING: Inconsistency between GPU and CPU result, check simulation \n!\n");
+    return e;
+}
+
 void Replica::DeviceMirror::reset()
 {
     if (ctx) remc_ctx_destroy(ctx);
@@ -690,10 +735,12 @@ void Replica::saveAsSinglePDB(const char *filename) const
 static void printHelp()
 {
     std::printf("Usage: remc_sim -f <filename> [-h] [-q] [-t x] [-s x] [-g x] [-m x] [-e x] [-r x] [-o x] "
-                "[-b x] [-t0 x] [-t1 x] [-bx x] [--seed x] [--device x] [--pid x] [--outdir d] [--data d]\n");
+                "[-b x] [-t0 x] [-t1 x] [-bx x] [-c] [--seed x] [--device x] [--pid x] [--outdir d] [--data d]\n");
     std::printf("\t-h|--help: show this dialog\n");
     std::printf("\t-f|--file <file>: Input config file\n");
     std::printf("\t-q|--nosim:       Do everything except the simulation\n");
+    std::printf("\t-c:              Check: the 10 example conformations summed by the plain and the batched kernel\n");
+    std::printf("\t--check-dir d:   directory of the example conformations conf1..conf10 (default: data)\n");
     std::printf("The following values override those in the config file\n");
     std::printf("\t-t|--threads x:  accepted for compatibility (one device context runs every replica)\n");
     std::printf("\t-s|--streams x:  accepted for compatibility\n");
@@ -747,7 +794,7 @@ bool getArgs(argdata *d, int argc, char **argv)
             ++i;
             continue;
         } else if (is("-c", nullptr)) {
-            std::printf("!!! -c is not supported; the test suite compares device energies with the oracle\n");
+            d->check = true;
             ++i;
             continue;
         }
@@ -774,6 +821,7 @@ bool getArgs(argdata *d, int argc, char **argv)
         else if (is("--outdir", nullptr)) d->outputDir = v;
         else if (is("--data", nullptr)) d->dataDir = v;
         else if (is("--save-pdb", nullptr)) d->savePdb = v;
+        else if (is("--check-dir", nullptr)) d->checkDir = v;
         else if (is("--final-pdb", nullptr)) d->finalPdb = v;
         else if (is("--coop", nullptr)) d->coop = std::atoi(v);
         else if (is("--checkpoint", nullptr)) d->checkpointFrequency = std::atoi(v);

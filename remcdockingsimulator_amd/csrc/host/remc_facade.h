@@ -120,6 +120,8 @@ struct argdata {
     int blockDim = 0; // MC workgroup size; 0 = autotuned
     bool dryRun = false;
     bool help = false;
+    bool check = false;            // -c: plain against batched kernel on the example conformations
+    std::string checkDir = "data"; // holds conf1/1a.pdb, conf1/1b.pdb, ... conf10/1b.pdb
     std::string savePdb, finalPdb; // initial / final (lowest temperature) conformation
     int coop = 0;                  // MC workgroups per replica; 0 = autotuned
     int checkpointFrequency = 0;   // MC steps between checkpoints; 0 = none
@@ -268,6 +270,17 @@ remc_forcefield makeForceField(const AminoAcids &aas);
 // (main.cpp:383-632).  Return false on a fatal error (message printed).
 bool getArgs(argdata *d, int argc, char **argv);
 bool loadArgsFromFile(argdata *d, Replica *initial);
+
+// -c (main.cpp:1652-1707): the ten example conformations <checkDir>/conf<i>/1a.pdb
+// + 1b.pdb in a 118.4 A box, each summed by the reference-shaped plain kernel
+// and by the batched full-energy kernel; prints one "PLAIN,BATCHED,err:" line
+// per conformation and returns 0, or 1 when a file or device call fails.
+int checkConformations(const AminoAcids &aas, const argdata &p);
+
+// The start-up consistency check (main.cpp:1767-1789): E of `initial` by the
+// batched kernel (returned, NAN on failure) against the plain kernel, with
+// the absolute and relative error and the reference's warning above 1 %.
+double checkedInitialEnergy(Replica *initial, int device);
 
 // The REMC driver with the reference's sampling and output files
 // (output/<prefix>_<pid>_{fractionBound,boundconformations,acceptance_ratios,

@@ -41,7 +41,7 @@ int main(int argc, char **argv)
         std::printf("The simulation will resume from the checkpoint file: %s\nOriginal input: %s\n",
                     p.resumeFile.c_str(), p.file.c_str());
     }
-    if (!p.inputFile) {
+    if (!p.inputFile && !p.check) {
         std::printf("! no configuration file given (-f); see -h\n");
         return 1;
     }
@@ -51,6 +51,7 @@ int main(int argc, char **argv)
     if (!aas.loadAminoAcidData((data + "/AminoAcids").c_str())) return 1;
     std::printf("Loading pair lookup table. %s/pair_potentials_MJ1996\n", data.c_str());
     if (!aas.loadLJPotentialData((data + "/pair_potentials_MJ1996").c_str())) return 1;
+    if (p.check) return checkConformations(aas, p); // main.cpp:1652-1707 exits after the check
 
     Replica initial;
     initial.setAminoAcidData(aas);
@@ -78,9 +79,8 @@ int main(int argc, char **argv)
         return 0;
     }
 
-    const double e = initial.E(p.device);
+    const double e = checkedInitialEnergy(&initial, p.device);
     if (e != e) return 1;
-    std::printf("GPU initial energy value: %26.23f kcal/mol\n", e);
     std::printf("RNG seed %llu\n", (unsigned long long)p.seed);
     std::printf("Output files will be prefixed by %s_%d\n", p.prependageString.c_str(), p.pid);
     std::vector<Replica> finals;
