@@ -75,6 +75,19 @@ __device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, fl
     return fminf(e, 1e30f);
 }
 
+// pair_lj_dh of an uncharged pair (qq = 0): the same number without the
+// Debye-Hueckel branch.
+__device__ __forceinline__ float pair_lj(float r2, float eij4, float sij2)
+{
+    float rinv = __builtin_amdgcn_rsqf(r2);
+    float s2 = sij2 * (rinv * rinv);
+    float x6 = s2 * s2 * s2;
+    float base = __builtin_fmaf(x6, x6, -x6);
+    bool flip = (eij4 > 0.0f) & (x6 > 0.5f);
+    float e = eij4 * (flip ? base + 0.5f : -base);
+    return fminf(e, 1e30f);
+}
+
 // Crowder term C (sigma_c / r)^6 inside the cutoff as crow_k (1/r^2)^3.  r2 = 0
 // (coincident beads) gives inf, capped like the pair term; no clamp is needed
 // since no zero multiplies the infinity.
@@ -461,7 +474,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         if (SPEC) {
             const unsigned long long run = ~(elig >> off);
             w = run == 0ull ? 64 : (int)__builtin_ctzll(run);
-            w = min(w, WK);
+            w = __builtin_amdgcn_readfirstlane(min(w, WK)); // the same in every lane
         }
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
@@ -538,12 +551,26 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             const unsigned long long wtag = step & 0x7FFFFull;
             unsigned long long *bank = slots + (n_x & 1u) * (REMC_SLOT_K * REMC_MAX_COOP);
             const int wjs = coop > 1 ? cjs : 0, wje = coop > 1 ? cje : N;
-            int k0 = wave;
-            if (wave + NW < w && s_draw[off + wave].nm == 1 && s_draw[off + wave + NW].nm == 1 &&
-                ((s_wtc[wave * SPEC_MAX] & s_wtc[(wave + NW) * SPEC_MAX]) >> 8) != 0) {
+            // Static slots: a one-bead body whose move leaves its position as it
+            // is (the rotation of a sphere) has dE = 0 exactly, with no partner
+            // loop, no partial to exchange and no correction.  The other
+            // (working) slots are dealt to the waves in order: ranks wave and
+            // wave + NW.
+            bool st_l = false;
+            if (lane < w && s_draw[off + lane].nm == 1) {
+                const float4 o1 = s_wold[lane * SPEC_MAX], n1 = s_wnew[lane * SPEC_MAX];
+                st_l = o1.x == n1.x && o1.y == n1.y && o1.z == n1.z;
+            }
+            const unsigned long long statm = __ballot(st_l);
+            const bool wk_l = lane < w && !st_l;
+            const int rank_l = __popcll(__ballot(wk_l) & ((1ull << lane) - 1ull));
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            const unsigned long long ma = __ballot(wk_l && rank_l == wv), mb = __ballot(wk_l && rank_l == wv + NW);
+            int ka = ma ? (int)__builtin_ctzll(ma) : WK, kb = mb ? (int)__builtin_ctzll(mb) : WK;
+            if (kb < w && s_draw[off + ka].nm == 1 && s_draw[off + kb].nm == 1 &&
+                ((s_wtc[ka * SPEC_MAX] & s_wtc[kb * SPEC_MAX]) >> 8) != 0) {
                 // both slots of the wave move one crowder bead: one pass over the
                 // partners serves both (the same per-lane order of terms as below)
-                const int ka = wave, kb = wave + NW;
                 const float4 oa = s_wold[ka * SPEC_MAX], na = s_wnew[ka * SPEC_MAX];
                 const float4 ob = s_wold[kb * SPEC_MAX], nb = s_wnew[kb * SPEC_MAX];
                 const int ba = s_draw[off + ka].m0, bb = s_draw[off + kb].m0;
@@ -575,9 +602,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         s_wbase[kb] = vb;
                     }
                 }
-                k0 = w; // both done
+                ka = kb = WK; // both done
             }
-            for (int k = k0; k < w; k += NW) {
+            for (int pass = 0; pass < 2; ++pass) {
+                const int k = pass == 0 ? ka : kb;
+                if (k >= w) continue;
                 const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
                 const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
                 float acc = 0.0f, comp = 0.0f;
@@ -614,6 +643,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
             }
             // pair corrections between the moved bodies of every two slots j < k
+            // (zero where either is static)
             if (tid < WK * WK) {
                 const int jj = tid / WK, kk = tid % WK;
                 if (jj < kk && kk < w) {
@@ -622,7 +652,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     // the same body: slot kk's proposal depends on slot jj's
                     // move, unless kk is chained to jj (the decision handles it)
                     bool touches = bj == bk && s_wchain[kk] == 0u;
-                    if (bj != bk) {
+                    if (bj != bk && ((statm >> jj) & 1ull) == 0ull && ((statm >> kk) & 1ull) == 0ull) {
                         const int nmj = s_draw[off + jj].nm, nmk = s_draw[off + kk].nm;
                         const bool crow = ((s_wtc[jj * SPEC_MAX] | s_wtc[kk * SPEC_MAX]) >> 8) != 0;
                         float acc = 0.0f, comp = 0.0f;
@@ -674,7 +704,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 #pragma unroll
                     for (int m = 0; m < REMC_MAX_COOP; ++m) {
                         v[m] = 0ull;
-                        ok[m] = lane >= w || m >= coop;
+                        ok[m] = lane >= w || m >= coop || st_l;
                     }
                     int spins = 0;
                     bool failed = false;
@@ -709,9 +739,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 #pragma unroll
                     for (int m = 0; m < REMC_MAX_COOP; ++m)
                         if (m < coop) dEl += slot_value(v[m]);
-                    if (lane >= w) dEl = 0.0;
+                    if (lane >= w || st_l) dEl = 0.0;
                 } else {
-                    dEl = lane < w ? s_wbase[lane] : 0.0;
+                    dEl = lane < w && !st_l ? s_wbase[lane] : 0.0;
                 }
                 // Every slot is decided at once on its dE0.  That is step k's
                 // exact dE unless an accepted earlier slot j of the window moved a
@@ -929,6 +959,41 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
             }
             const bool cr0 = crow_m || (tcj[0] >> 8) != 0, cr1 = crow_m || (tcj[1] >> 8) != 0;
+            if (__all(!cr0 && !(on[1] && cr1))) {
+                // Every active lane's partners are non-crowder beads (a partner
+                // that is off repeats partner 0 and is masked out), the common
+                // case: the four terms of a moved bead (old and new position
+                // against both partners) are formed side by side with no
+                // per-lane branch, four independent chains per wave.  The
+                // Debye-Hueckel part is skipped as a whole when no moved bead
+                // of the wave is charged.
+                const int t0 = tcj[0] & 0xff, t1 = tcj[1] & 0xff;
+                for (int k = k0; k < k1; ++k) {
+                    const float4 o = s_old[k], n = s_new[k];
+                    const float *erow = s_etab + (s_tc[k] & 0xff) * REMC_NTYPES;
+                    const float e0 = erow[t0], e1 = erow[t1];
+                    const float h0 = 0.5f * (n.w + sj[0]), h1 = 0.5f * (n.w + sj[1]);
+                    const float ss0 = h0 * h0, ss1 = h1 * h1;
+                    const float ro0 = dist2(xj[0], yj[0], zj[0], o, S), rn0 = dist2(xj[0], yj[0], zj[0], n, S);
+                    const float ro1 = dist2(xj[1], yj[1], zj[1], o, S), rn1 = dist2(xj[1], yj[1], zj[1], n, S);
+                    float eo0, en0, eo1, en1;
+                    if (__any(o.w != 0.0f)) {
+                        const float qq0 = o.w * qj[0], qq1 = o.w * qj[1];
+                        eo0 = pair_lj_dh<false>(ro0, e0, ss0, qq0, ff);
+                        en0 = pair_lj_dh<false>(rn0, e0, ss0, qq0, ff);
+                        eo1 = pair_lj_dh<false>(ro1, e1, ss1, qq1, ff);
+                        en1 = pair_lj_dh<false>(rn1, e1, ss1, qq1, ff);
+                    } else {
+                        eo0 = pair_lj(ro0, e0, ss0);
+                        en0 = pair_lj(rn0, e0, ss0);
+                        eo1 = pair_lj(ro1, e1, ss1);
+                        en1 = pair_lj(rn1, e1, ss1);
+                    }
+                    kahan_add(acc, comp, en0 - eo0);
+                    kahan_add(acc, comp, on[1] ? en1 - eo1 : 0.0f);
+                }
+                continue;
+            }
             for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], n = s_new[k];
                 const int ti = s_tc[k] & 0xff;
