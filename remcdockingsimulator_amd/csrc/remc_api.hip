@@ -447,6 +447,7 @@ int finalize(remc_ctx *c)
     c->devff.c_crow = (float)lj_scale;
     c->devff.crow_s2 = (float)(f.crowder_sigma * f.crowder_sigma);
     c->devff.crow_cut2 = (float)(f.crowder_cutoff * f.crowder_cutoff);
+    c->devff.crow_cut = (float)f.crowder_cutoff;
     c->devff.c_dh = (float)(f.dh_cf * f.kbt_cf);
     c->devff.k_dh = (float)(-1.4426950408889634 / f.xi);
     c->devff.eps_r = 1e-38f;

@@ -47,7 +47,7 @@ struct DevFF {
     float k_dh;      // -log2(e)/xi, exp(-r/xi) = exp2(r*k_dh)
     float eps_r;     // 1e-38 added to r
     float crow_k;    // c_crow * crow_s2^3: crowder term = crow_k / r^6
-    float pad1;
+    float crow_cut;  // crowder cutoff
 };
 
 struct KParams {

@@ -98,6 +98,18 @@ __device__ __forceinline__ float pair_crowder(float r2, const DevFF &ff)
     return r2 < ff.crow_cut2 ? fminf(ff.crow_k * rc3, 1e30f) : 0.0f;
 }
 
+// Squared screening radius of a one-bead crowder move from o to n: a partner
+// bead farther than the cutoff plus the move's length (+0.01 A for the FP32
+// rounding of the distances) from o is beyond the cutoff of both positions,
+// so both of its crowder terms are exactly 0.
+__device__ __forceinline__ float crowder_screen2(float4 o, float4 n, const DevSys &S, const DevFF &ff)
+{
+    const float dx = min_image(n.x - o.x, S.box, S.inv_box), dy = min_image(n.y - o.y, S.box, S.inv_box),
+                dz = min_image(n.z - o.z, S.box, S.inv_box);
+    const float lim = ff.crow_cut + sqrtf(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz))) + 0.01f;
+    return lim * lim;
+}
+
 // Kahan step, kept as written (no reassociation is enabled for this file).
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v)
 {
@@ -574,17 +586,22 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 const float4 oa = s_wold[ka * SPEC_MAX], na = s_wnew[ka * SPEC_MAX];
                 const float4 ob = s_wold[kb * SPEC_MAX], nb = s_wnew[kb * SPEC_MAX];
                 const int ba = s_draw[off + ka].m0, bb = s_draw[off + kb].m0;
+                const float lim2a = crowder_screen2(oa, na, S, ff), lim2b = crowder_screen2(ob, nb, S, ff);
                 float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
                 for (int j = wjs + lane; j < wje; j += 64) {
                     const int jc = j - clo;
                     const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                    // the terms at the new positions only where a partner of the
+                    // wave is near enough to give any (crowders are sparse)
+                    const float r2a = dist2(xj, yj, zj, oa, S), r2b = dist2(xj, yj, zj, ob, S);
+                    if (!__any((j != ba && r2a < lim2a) || (j != bb && r2b < lim2b))) continue;
                     if (j != ba) {
-                        const float eo = pair_crowder(dist2(xj, yj, zj, oa, S), ff);
+                        const float eo = pair_crowder(r2a, ff);
                         const float en = pair_crowder(dist2(xj, yj, zj, na, S), ff);
                         kahan_add(acc_a, comp_a, en - eo);
                     }
                     if (j != bb) {
-                        const float eo = pair_crowder(dist2(xj, yj, zj, ob, S), ff);
+                        const float eo = pair_crowder(r2b, ff);
                         const float en = pair_crowder(dist2(xj, yj, zj, nb, S), ff);
                         kahan_add(acc_b, comp_b, en - eo);
                     }
@@ -610,7 +627,23 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 const int m0k = s_draw[off + k].m0, nmk = s_draw[off + k].nm, m1k = m0k + nmk;
                 const bool crow_m = (s_wtc[k * SPEC_MAX] >> 8) != 0;
                 float acc = 0.0f, comp = 0.0f;
-                for (int i = 0; i < nmk; ++i) {
+                if (nmk == 1 && crow_m) {
+                    // one crowder bead: partners screened as in the two-slot pass
+                    const float4 o = s_wold[k * SPEC_MAX], n = s_wnew[k * SPEC_MAX];
+                    const float lim2 = crowder_screen2(o, n, S, ff);
+                    for (int j = wjs + lane; j < wje; j += 64) {
+                        const int jc = j - clo;
+                        const float xj = s_jx[jc], yj = s_jy[jc], zj = s_jz[jc];
+                        const float r2o = dist2(xj, yj, zj, o, S);
+                        if (!__any(j != m0k && r2o < lim2)) continue;
+                        if (j != m0k) {
+                            const float eo = pair_crowder(r2o, ff);
+                            const float en = pair_crowder(dist2(xj, yj, zj, n, S), ff);
+                            kahan_add(acc, comp, en - eo);
+                        }
+                    }
+                }
+                for (int i = 0; i < (nmk == 1 && crow_m ? 0 : nmk); ++i) {
                     const float4 o = s_wold[k * SPEC_MAX + i], n = s_wnew[k * SPEC_MAX + i];
                     const float *erow = s_etab + (s_wtc[k * SPEC_MAX + i] & 0xff) * REMC_NTYPES;
                     for (int j = wjs + lane; j < wje; j += 64) {
