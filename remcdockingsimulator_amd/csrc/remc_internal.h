@@ -146,5 +146,6 @@ constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC
 constexpr int REMC_SLOT_K = 64;  // partial sums per member and exchange (MC window slots, force-biased window values)
 // two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica
 constexpr int REMC_SLOTS_PER_REP = 2 * REMC_SLOT_K * REMC_MAX_COOP;
+static_assert(REMC_SLOT_K >= 64, "every lane of a polling wave reads a slot row of its own");
 
 } // namespace remc

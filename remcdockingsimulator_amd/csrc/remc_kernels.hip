@@ -732,27 +732,23 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 if (coop > 1) {
                     // lane k gathers slot k's coop partials (bank[k][m]) and adds
                     // them in member order: the same total in every member
+                    // Every lane reads all REMC_MAX_COOP entries of its slot row
+                    // (the row exists for every lane: REMC_SLOT_K >= 64) with no
+                    // branch between the loads, so a poll costs one memory round
+                    // trip, not one per member.
                     unsigned long long v[REMC_MAX_COOP];
-                    bool ok[REMC_MAX_COOP];
-#pragma unroll
-                    for (int m = 0; m < REMC_MAX_COOP; ++m) {
-                        v[m] = 0ull;
-                        ok[m] = lane >= w || m >= coop || st_l;
-                    }
                     int spins = 0;
                     bool failed = false;
                     for (;;) {
-                        bool mine = true;
-#pragma unroll
-                        for (int m = 0; m < REMC_MAX_COOP; ++m) mine = mine && ok[m];
-                        if (__all(mine)) break;
 #pragma unroll
                         for (int m = 0; m < REMC_MAX_COOP; ++m)
-                            if (!ok[m]) {
-                                v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-                                ok[m] = (v[m] & SLOT_TAG_MASK) == wtag;
-                            }
+                            v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                        bool mine = true;
+#pragma unroll
+                        for (int m = 0; m < REMC_MAX_COOP; ++m)
+                            mine = mine && (m >= coop || (v[m] & SLOT_TAG_MASK) == wtag);
+                        if (__all(mine || lane >= w || st_l)) break;
                         if (++spins >= SLOT_SPIN_LIMIT) {
                             failed = true;
                             break;
@@ -1366,27 +1362,21 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
             if (lane < n)
                 __hip_atomic_store(bank + lane * REMC_MAX_COOP + c, slot_pack(vals[lane], tag), __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_AGENT);
+            // every lane reads its whole row (REMC_SLOT_K >= 64 rows) with no
+            // branch between the loads: one memory round trip per poll
             unsigned long long v[REMC_MAX_COOP];
-            bool ok[REMC_MAX_COOP];
-#pragma unroll
-            for (int m = 0; m < REMC_MAX_COOP; ++m) {
-                v[m] = 0ull;
-                ok[m] = lane >= n || m >= coop;
-            }
             int spins = 0;
             bool failed = false;
             for (;;) {
-                bool mine = true;
-#pragma unroll
-                for (int m = 0; m < REMC_MAX_COOP; ++m) mine = mine && ok[m];
-                if (__all(mine)) break;
 #pragma unroll
                 for (int m = 0; m < REMC_MAX_COOP; ++m)
-                    if (!ok[m]) {
-                        v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-                        ok[m] = (v[m] & SLOT_TAG_MASK) == tag;
-                    }
+                    v[m] = __hip_atomic_load(bank + lane * REMC_MAX_COOP + m, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
+                bool mine = true;
+#pragma unroll
+                for (int m = 0; m < REMC_MAX_COOP; ++m)
+                    mine = mine && (m >= coop || (v[m] & SLOT_TAG_MASK) == tag);
+                if (__all(mine || lane >= n)) break;
                 if (++spins >= SLOT_SPIN_LIMIT) {
                     failed = true;
                     break;
