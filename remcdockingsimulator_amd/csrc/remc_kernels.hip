@@ -304,6 +304,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ unsigned int s_wchain[WK];
     __shared__ int s_wnext[WK]; // the accepted slot chained to slot j, or -1
     __shared__ int s_wdec[WK];
+    __shared__ int s_wstat[WK]; // 1: the slot is static (dE = 0, nothing to evaluate)
     __shared__ int s_weff;
     __shared__ double s_E;
     __shared__ unsigned long long s_cnt[3];
@@ -535,6 +536,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         const float qb = inc ? s_jq[b - clo] : qs[b];
                         s_wold[tid] = make_float4(ox, oy, oz, qb * ff.c_dh);
                         s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+                        // a static slot: one bead whose move leaves it where it is
+                        if (i == 0)
+                            s_wstat[k] = nmk == 1 && ox == nrx + ncx && oy == nry + ncy && oz == nrz + ncz ? 1 : 0;
                         s_wrel[tid] = make_float4(nrx, nry, nrz, qb);
                         s_wtc[tid] = inc ? s_jtc[b - clo] : tcs[b];
                         if (i == 0) {
@@ -568,11 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             // loop, no partial to exchange and no correction.  The other
             // (working) slots are dealt to the waves in order: ranks wave and
             // wave + NW.
-            bool st_l = false;
-            if (lane < w && s_draw[off + lane].nm == 1) {
-                const float4 o1 = s_wold[lane * SPEC_MAX], n1 = s_wnew[lane * SPEC_MAX];
-                st_l = o1.x == n1.x && o1.y == n1.y && o1.z == n1.z;
-            }
+            const bool st_l = lane < w && s_wstat[lane] != 0;
             const unsigned long long statm = __ballot(st_l);
             const bool wk_l = lane < w && !st_l;
             const int rank_l = __popcll(__ballot(wk_l) & ((1ull << lane) - 1ull));
