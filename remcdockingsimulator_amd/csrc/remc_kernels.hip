@@ -1266,6 +1266,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
     __shared__ double s_wdisp[3 * FBW], s_wfwd[FBW], s_wqp[4 * FBW], s_wdE[FBW];
     __shared__ unsigned int s_wint[FBW];
     __shared__ int s_wdec[FBW], s_weff;
+    __shared__ int s_wstat[FBW]; // 1: the slot leaves its bead where it is (the rotation of a sphere)
     extern __shared__ float4 smem[];
     float4 *s_old = smem;                // old position, q_i c_dh
     float4 *s_orel = s_old + p.lds_mb;   // old relative position, sigma
@@ -1464,9 +1465,17 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 const int ba = s_fm0[bw + ka], bb = two ? s_fm0[bw + kb] : ba;
                 const float ax = s_jx[ba], ay = s_jy[ba], az = s_jz[ba];
                 const float bx = s_jx[bb], by = s_jy[bb], bz = s_jz[bb];
+                // partners beyond the cutoff (+0.01 A for rounding) give exactly
+                // zero force: a wave with none nearer skips the derivative
+                const float limA = (ff.crow_cut + 0.01f) * (ff.crow_cut + 0.01f);
                 double fa0 = 0.0, fa1 = 0.0, fa2 = 0.0, fb0 = 0.0, fb1 = 0.0, fb2 = 0.0;
                 for (int j = js + lane; j < je; j += 64) {
                     const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
+                    {
+                        const float4 pa = make_float4(ax, ay, az, 0.0f), pb = make_float4(bx, by, bz, 0.0f);
+                        const float r2a = dist2(xj, yj, zj, pa, S), r2b = dist2(xj, yj, zj, pb, S);
+                        if (!__any((j != ba && r2a < limA) || (two && j != bb && r2b < limA))) continue;
+                    }
                     if (j != ba) {
                         const float dx = min_image(ax - xj, S.box, S.inv_box);
                         const float dy = min_image(ay - yj, S.box, S.inv_box);
@@ -1552,6 +1561,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 if (lane == 0) {
                     s_wold[k] = make_float4(s_jx[b], s_jy[b], s_jz[b], 0.0f);
                     s_wnew[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, 0.0f);
+                    s_wstat[k] = s_jx[b] == nrx + ncx && s_jy[b] == nry + ncy && s_jz[b] == nrz + ncz ? 1 : 0;
                     s_wnrel[k] = make_float4(nrx, nry, nrz, 0.0f);
                     s_wncen[k] = make_float4(ncx, ncy, ncz, 0.0f);
                     for (int t = 0; t < 3; ++t) s_wdisp[3 * k + t] = disp[t];
@@ -1566,7 +1576,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 if (lane < k) {
                     if (s_fbd[bw + lane].body == s_fbd[bw + k].body) {
                         touch = true;
-                    } else {
+                    } else if (s_wstat[lane] == 0 && s_wstat[k] == 0) {
+                        // a static slot's bead stays put: its dE is 0 and its
+                        // torque 0 whatever moves around it, and it moves nothing
                         const float4 oj = s_wold[lane], nj_ = s_wnew[lane], ok = s_wold[k], nk = s_wnew[k];
                         touch = dist2(oj.x, oj.y, oj.z, ok, S) < ff.crow_cut2 || dist2(oj.x, oj.y, oj.z, nk, S) < ff.crow_cut2 ||
                                 dist2(nj_.x, nj_.y, nj_.z, ok, S) < ff.crow_cut2 ||
@@ -1576,16 +1588,31 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 const unsigned long long tm = __ballot(touch);
                 if (lane == 0) s_wint[k] = (unsigned int)tm;
             }
-            if (wave < w) {
+            // static slots: dE = 0 and the new force is the old one (the bead
+            // is where it was); their entries are zero partials, and the
+            // decision takes the old force.  The other slots are dealt to the
+            // waves in order, two per wave.
+            const bool wk_l = lane < w && s_wstat[lane] == 0;
+            const int rank_l = __popcll(__ballot(wk_l) & ((1ull << lane) - 1ull));
+            const int wv = __builtin_amdgcn_readfirstlane(wave);
+            const unsigned long long mka = __ballot(wk_l && rank_l == wv), mkb = __ballot(wk_l && rank_l == wv + NW);
+            if (tid < w && s_wstat[tid] != 0)
+                for (int t = 0; t < 4; ++t) s_wB[4 * tid + t] = 0.0;
+            if (mka != 0ull) {
                 // dE and the force at the new position, slots ka and kb in one pass
-                const int ka = wave, kb = wave + NW;
-                const bool two = kb < w;
+                const int ka = (int)__builtin_ctzll(mka), kb = mkb != 0ull ? (int)__builtin_ctzll(mkb) : ka;
+                const bool two = mkb != 0ull;
                 const int ba = s_fm0[bw + ka], bb = two ? s_fm0[bw + kb] : ba;
                 const float4 oa = s_wold[ka], na = s_wnew[ka], ob = s_wold[two ? kb : ka], nb = s_wnew[two ? kb : ka];
+                const float limBa = crowder_screen2(oa, na, S, ff), limBb = crowder_screen2(ob, nb, S, ff);
                 float acc_a = 0.0f, comp_a = 0.0f, acc_b = 0.0f, comp_b = 0.0f;
                 double fa0 = 0.0, fa1 = 0.0, fa2 = 0.0, fb0 = 0.0, fb1 = 0.0, fb2 = 0.0;
                 for (int j = js + lane; j < je; j += 64) {
                     const float xj = s_jx[j], yj = s_jy[j], zj = s_jz[j];
+                    {
+                        const float r2a = dist2(xj, yj, zj, oa, S), r2b = dist2(xj, yj, zj, ob, S);
+                        if (!__any((j != ba && r2a < limBa) || (two && j != bb && r2b < limBb))) continue;
+                    }
                     if (j != ba) {
                         float eo, en, go, g2;
                         {
@@ -1661,7 +1688,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                     const int k = lane;
                     const remc_fb_draws dk = s_fbd[bw + k];
                     const float4 nr = s_wnrel[k];
-                    const double Fn[3] = {s_wB[4 * k + 1], s_wB[4 * k + 2], s_wB[4 * k + 3]};
+                    const bool stk = s_wstat[k] != 0;
+                    const double Fn[3] = {stk ? s_wA[3 * k] : s_wB[4 * k + 1], stk ? s_wA[3 * k + 1] : s_wB[4 * k + 2],
+                                          stk ? s_wA[3 * k + 2] : s_wB[4 * k + 3]};
                     const double tn[3] = {(double)nr.y * Fn[2] - (double)nr.z * Fn[1],
                                           (double)nr.z * Fn[0] - (double)nr.x * Fn[2],
                                           (double)nr.x * Fn[1] - (double)nr.y * Fn[0]};
