@@ -634,36 +634,46 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
     p.cache_rel = 0;
     p.cache_quat = 0;
     p.spec = 0;
+    p.draw2 = 0;
     if (!c->lds_cache || budget <= 0) return;
     const long long full = 24LL * c->max_beads;
     const bool all_split = coop > 1 && c->min_move_pairs >= c->coop_min_pairs;
     const long long slice = 24LL * ((c->max_beads + coop - 1) / coop + 1);
-    if (full <= budget) {
-        p.lds_cl = c->max_beads;
-        budget -= full;
-    } else if (all_split && slice <= budget) {
-        p.lds_cl = (c->max_beads + coop - 1) / coop + 1;
-        p.cache_slice = 1;
-        budget -= slice;
-    } else {
-        return;
-    }
-    // speculative windows read the partner range (the replica, or the member's slice) from LDS
-    p.spec = c->mc_window ? 1 : 0;
-    if (20LL * c->max_bodies <= budget) {
-        p.lds_nbody = c->max_bodies;
-        p.cache_bodies = 1;
-        budget -= 20LL * c->max_bodies;
-    } else {
-        return;
-    }
-    if (!p.cache_slice && 12LL * c->max_beads <= budget) {
-        p.cache_rel = 1;
-        budget -= 12LL * c->max_beads;
-    }
-    if (32LL * c->max_bodies <= budget) {
-        p.cache_quat = 1;
-        budget -= 32LL * c->max_bodies;
+    [&] {
+        if (full <= budget) {
+            p.lds_cl = c->max_beads;
+            budget -= full;
+        } else if (all_split && slice <= budget) {
+            p.lds_cl = (c->max_beads + coop - 1) / coop + 1;
+            p.cache_slice = 1;
+            budget -= slice;
+        } else {
+            return;
+        }
+        // speculative windows read the partner range (the replica, or the member's slice) from LDS
+        p.spec = c->mc_window ? 1 : 0;
+        if (20LL * c->max_bodies <= budget) {
+            p.lds_nbody = c->max_bodies;
+            p.cache_bodies = 1;
+            budget -= 20LL * c->max_bodies;
+        } else {
+            return;
+        }
+        if (!p.cache_slice && 12LL * c->max_beads <= budget) {
+            p.cache_rel = 1;
+            budget -= 12LL * c->max_beads;
+        }
+        if (32LL * c->max_bodies <= budget) {
+            p.cache_quat = 1;
+            budget -= 32LL * c->max_bodies;
+        }
+    }();
+    // a second draw batch, formed ahead by waves the window staging leaves
+    // idle (workgroups of 8 waves or more), with what LDS the caches leave
+    const long long draws = (long long)mc_draw_batch_bytes();
+    if (p.spec && block >= 512 && draws <= budget) {
+        p.draw2 = 1;
+        budget -= draws;
     }
 }
 

@@ -91,6 +91,7 @@ struct KParams {
     int cache_rel;          // 1: relative positions cached (full partner cache only)
     int cache_quat;         // 1: body rotations cached (with the body table)
     int spec;               // 1: speculative windows of small moves (needs the full caches)
+    int draw2;              // 1: a second draw batch in dynamic LDS, formed ahead
     // recording
     remc_step_record *rec;
     unsigned long long *rec_start;
@@ -103,6 +104,7 @@ hipError_t launch_mc(const KParams &p, int n_rep, int nsteps, int block, int coo
 size_t mc_dynamic_lds(const KParams &p);
 // static LDS of the MC kernel at this block size (its largest variant), or -1
 int mc_static_lds(int block);
+size_t mc_draw_batch_bytes(); // LDS of one batch of MC step draws
 hipError_t launch_mc_fb(const KParams &p, int n_rep, int nsteps, int block, int coop, hipStream_t s);
 hipError_t mc_fb_max_active_per_cu(const KParams &p, int block, int *out);
 size_t mc_fb_dynamic_lds(const KParams &p);

@@ -246,6 +246,7 @@ __device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, c
 }
 
 // Dynamic LDS of the MC kernel, carved in this order (mc_dynamic_lds sizes it):
+//   draw2:                  StepDraw[DRAW_BATCH]              the draw batch formed ahead
 //   cache_quat:             double q[4*nbody]                 body rotations
 //   always:                 float4 old[mb], new[mb]; int tc[mb] moved-body staging
 //   CACHE:                  float x,y,z,q,sig[cl]; int tc[cl]  partner beads [clo, clo+cl)
@@ -256,12 +257,15 @@ size_t mc_dynamic_lds(const KParams &p)
 {
     const size_t mb = p.lds_mb, cl = p.lds_cl, nb = p.lds_nbody;
     size_t s = 36 * mb;
+    if (p.draw2) s += mc_draw_batch_bytes();
     if (p.cache_quat) s += 32 * nb;
     if (cl) s += 24 * cl;
     if (cl && p.cache_rel) s += 12 * cl;
     if (cl && p.cache_bodies) s += 20 * nb;
     return s;
 }
+
+size_t mc_draw_batch_bytes() { return sizeof(StepDraw) * DRAW_BATCH; }
 
 // One launch runs nsteps MC steps of every replica.  A step whose moved body
 // is small (at most SPEC_MAX beads) may be taken in a speculative window when
@@ -291,8 +295,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
-    __shared__ StepDraw s_draw[DRAW_BATCH];
-    __shared__ unsigned long long s_elig; // window-eligible steps of the draw batch
+    __shared__ StepDraw s_draw0[DRAW_BATCH];
+    __shared__ unsigned long long s_elig[2]; // window-eligible steps of each draw batch
     __shared__ float4 s_wold[WS], s_wnew[WS], s_wrel[WS]; // old (w: q c_dh) / new (w: sigma) position, new rel (w: q)
     __shared__ int s_wtc[WS];                             // type | crowder << 8 of the staged beads
     __shared__ float4 s_wcen[WK];                         // new centre of a translated slot
@@ -311,6 +315,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     extern __shared__ float4 smem[];
 
     char *lp = (char *)smem;
+    StepDraw *s_draw1 = (StepDraw *)lp; // the second batch (p.draw2)
+    lp += p.draw2 ? sizeof(StepDraw) * DRAW_BATCH : 0;
     double *s_q = (double *)lp;
     lp += p.cache_quat ? 32 * (size_t)p.lds_nbody : 0;
     float4 *s_old = (float4 *)lp;
@@ -395,6 +401,12 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 
     int b0 = 0, have = 0; // s_draw holds the draws of steps [b0, b0 + have) of this launch
     unsigned long long elig = 0ull; // s_elig of the batch, read once per batch
+    // With p.draw2 the draws of steps [nb0, nb0 + DRAW_BATCH), nb0 = b0 +
+    // DRAW_BATCH/2, are formed ahead into the other buffer by four waves that
+    // a window's staging leaves idle; the refill then only switches buffers.
+    int dbuf = 0;        // the buffer of the current batch
+    bool nb_ok = false;  // the other buffer holds the batch from nb0
+    int nb0 = 0;
     int it = 0;
     while (it < nsteps) {
         // The thread index is re-read opaquely in every step, so that the
@@ -414,74 +426,89 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             const unsigned long long rest = (1ull << (have - off)) - 1ull; // have - off < WK <= 32
             refill = ((~(elig >> off)) & rest) == 0ull;
         }
-        if (refill) {
-            // the draws of the next DRAW_BATCH steps, one step per lane, formed
-            // by four waves side by side: each forms the counter words itself
-            // and then its part (wave 0 the body, its bead range, the window
-            // eligibility and the same-body links; wave 1 a rotation's
-            // quaternion and coefficients; wave 2 a translation; wave 3 -ln u)
-            if (wave < 4) {
-                const bool valid = it + lane < nsteps;
-                remc_draws dd{};
-                if (valid) dd = remc_step_draws(step + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
-                StepDraw &sd = s_draw[lane];
-                if (wave == 0) {
-                    bool el = false;
-                    const int body_l = valid ? dd.body : -1;
-                    if (valid) {
-                        sd.body = dd.body;
-                        sd.rotate = dd.rotate;
-                        const int gb = S.body_off + dd.body;
-                        sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
-                        sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
-                        if (SPEC) el = sd.nm <= SPEC_MAX;
-                    }
-                    if (SPEC) {
-                        const unsigned long long m = __ballot(el);
-                        if (lane == 0) s_elig = m;
-                        // the latest two earlier steps of the batch that move the same
-                        // body: the lanes whose body index agrees with this lane's in
-                        // every bit (one ballot per bit); lanes past the launch's last
-                        // step lie above every valid lane, so never count as earlier
-                        const int nbits = S.n_bodies > 1 ? 32 - __builtin_clz((unsigned int)(S.n_bodies - 1)) : 0;
-                        unsigned long long same = ~0ull;
-                        for (int bt = 0; bt < nbits; ++bt) {
-                            const bool on = ((body_l >> bt) & 1) != 0;
-                            const unsigned long long mb = __ballot(on);
-                            same &= on ? mb : ~mb;
-                        }
-                        const unsigned long long below = same & ((1ull << lane) - 1ull);
-                        int p1 = -1, p2 = -1;
-                        if (below != 0ull) {
-                            p1 = 63 - __builtin_clzll(below);
-                            const unsigned long long rest = below & ~(1ull << p1);
-                            if (rest != 0ull) p2 = 63 - __builtin_clzll(rest);
-                        }
-                        if (valid) {
-                            sd.prev1 = p1;
-                            sd.prev2 = p2;
-                        }
-                    }
-                } else if (wave == 1) {
-                    if (valid && dd.rotate) {
-                        double qq[4], cf[9];
-                        remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
-                        remc_qcoeffs(qq, cf);
-                        for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
-                        for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
-                    }
-                } else if (wave == 2) {
-                    if (valid && !dd.rotate) remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
-                } else {
-                    if (valid) sd.nlu = remc_neglog_u(dd.u);
+        // The draws of DRAW_BATCH steps from launch step base (RNG counter
+        // cbase), one step per lane, into dst, formed by four waves side by
+        // side (role 0..3): each forms the counter words itself and then its
+        // part (role 0 the body, its bead range, the window eligibility and the
+        // same-body links; role 1 a rotation's quaternion and coefficients;
+        // role 2 a translation; role 3 -ln u).
+        auto form_draws = [&](StepDraw *dst, int eb, int base, unsigned long long cbase, int role) {
+            if (role < 0 || role >= 4) return;
+            const bool valid = base + lane < nsteps;
+            remc_draws dd{};
+            if (valid) dd = remc_step_draws(cbase + lane, R.rng_id, p.key0, p.key1, S.n_bodies);
+            StepDraw &sd = dst[lane];
+            if (role == 0) {
+                bool el = false;
+                const int body_l = valid ? dd.body : -1;
+                if (valid) {
+                    sd.body = dd.body;
+                    sd.rotate = dd.rotate;
+                    const int gb = S.body_off + dd.body;
+                    sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
+                    sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
+                    if (SPEC) el = sd.nm <= SPEC_MAX;
                 }
+                if (SPEC) {
+                    const unsigned long long m = __ballot(el);
+                    if (lane == 0) s_elig[eb] = m;
+                    // the latest two earlier steps of the batch that move the same
+                    // body: the lanes whose body index agrees with this lane's in
+                    // every bit (one ballot per bit); lanes past the launch's last
+                    // step lie above every valid lane, so never count as earlier
+                    const int nbits = S.n_bodies > 1 ? 32 - __builtin_clz((unsigned int)(S.n_bodies - 1)) : 0;
+                    unsigned long long same = ~0ull;
+                    for (int bt = 0; bt < nbits; ++bt) {
+                        const bool on = ((body_l >> bt) & 1) != 0;
+                        const unsigned long long mb = __ballot(on);
+                        same &= on ? mb : ~mb;
+                    }
+                    const unsigned long long below = same & ((1ull << lane) - 1ull);
+                    int p1 = -1, p2 = -1;
+                    if (below != 0ull) {
+                        p1 = 63 - __builtin_clzll(below);
+                        const unsigned long long rest = below & ~(1ull << p1);
+                        if (rest != 0ull) p2 = 63 - __builtin_clzll(rest);
+                    }
+                    if (valid) {
+                        sd.prev1 = p1;
+                        sd.prev2 = p2;
+                    }
+                }
+            } else if (role == 1) {
+                if (valid && dd.rotate) {
+                    double qq[4], cf[9];
+                    remc_move_quat(dd.a, dd.b, dd.c, R.rot_ch, R.rot_sh, qq);
+                    remc_qcoeffs(qq, cf);
+                    for (int i = 0; i < 4; ++i) sd.q[i] = qq[i];
+                    for (int i = 0; i < 9; ++i) sd.coef[i] = cf[i];
+                }
+            } else if (role == 2) {
+                if (valid && !dd.rotate) remc_move_translation(dd.a, dd.b, dd.c, R.tstep, &sd.t[0], &sd.t[1], &sd.t[2]);
+            } else {
+                if (valid) sd.nlu = remc_neglog_u(dd.u);
             }
-            b0 = it;
-            have = min(DRAW_BATCH, nsteps - it);
-            off = 0;
-            __syncthreads();
-            if (SPEC) elig = s_elig;
+        };
+        if (refill) {
+            if (nb_ok && it >= nb0) {
+                // the batch formed ahead covers this step and the rest of the old one
+                dbuf ^= 1;
+                b0 = nb0;
+                have = min(DRAW_BATCH, nsteps - nb0);
+                off = it - b0;
+                nb_ok = false;
+                if (SPEC) elig = s_elig[dbuf];
+            } else {
+                form_draws(dbuf ? s_draw1 : s_draw0, dbuf, it, step, wave);
+                b0 = it;
+                have = min(DRAW_BATCH, nsteps - it);
+                off = 0;
+                nb_ok = false;
+                __syncthreads();
+                if (SPEC) elig = s_elig[dbuf];
+            }
         }
+        StepDraw *const s_draw = dbuf ? s_draw1 : s_draw0;
 
         int w = 0;
         if (SPEC) {
@@ -492,7 +519,14 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
             // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
+            // (waves 0 .. NW/2 - 1); waves NW/2 .. NW/2 + 3 meanwhile form the
+            // next draw batch when it is due
             if (tid < WK) s_wint[tid] = 0u;
+            if (NW >= 8 && p.draw2 && !nb_ok && off >= DRAW_BATCH / 2 && b0 + DRAW_BATCH / 2 < nsteps) {
+                nb0 = b0 + DRAW_BATCH / 2;
+                form_draws(dbuf ? s_draw0 : s_draw1, dbuf ^ 1, nb0, step - (unsigned long long)(it - nb0), wave - NW / 2);
+                nb_ok = true;
+            }
             if (tid < WK * SPEC_MAX) {
                 const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
                 if (k < w) {
