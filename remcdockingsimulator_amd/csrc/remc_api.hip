@@ -626,7 +626,7 @@ void fill_mc_lds(remc_ctx *c, KParams &p, int coop, int block)
     const int static_lds = (kernel_lds > 0 ? kernel_lds : 48 * 1024) + 2048;
     long long budget = std::min<long long>(c->lds_per_block, c->lds_per_cu / per_cu) - static_lds;
     p.lds_mb = std::max(1, c->max_body_beads);
-    budget -= 36LL * p.lds_mb;
+    budget -= 68LL * p.lds_mb;
     p.lds_cl = 0;
     p.lds_nbody = 0;
     p.cache_slice = 0;

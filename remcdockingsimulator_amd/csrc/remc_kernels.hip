@@ -75,9 +75,23 @@ __device__ __forceinline__ float pair_lj_dh(float r2, float eij4, float sij2, fl
     return fminf(e, 1e30f);
 }
 
-// pair_lj_dh of an uncharged pair (qq = 0): the same number without the
-// Debye-Hueckel branch.
+// pair_lj_dh of an uncharged pair (qq = 0) without the Debye-Hueckel branch,
+// with (sigma/r)^2 from one reciprocal instead of a squared reciprocal root
+// (the two differ in the last bits; a difference en - eo is always formed
+// from one of the two forms).
 __device__ __forceinline__ float pair_lj(float r2, float eij4, float sij2)
+{
+    float s2 = sij2 * __builtin_amdgcn_rcpf(r2);
+    float x6 = s2 * s2 * s2;
+    float base = __builtin_fmaf(x6, x6, -x6);
+    bool flip = (eij4 > 0.0f) & (x6 > 0.5f);
+    float e = eij4 * (flip ? base + 0.5f : -base);
+    return fminf(e, 1e30f);
+}
+
+// pair_lj_dh of every pair, charged or not, with the Debye-Hueckel exponent
+// scale given (k_dh times the length unit of r2).
+__device__ __forceinline__ float pair_lj_dh_k(float r2, float eij4, float sij2, float qq, float kdh)
 {
     float rinv = __builtin_amdgcn_rsqf(r2);
     float s2 = sij2 * (rinv * rinv);
@@ -85,6 +99,7 @@ __device__ __forceinline__ float pair_lj(float r2, float eij4, float sij2)
     float base = __builtin_fmaf(x6, x6, -x6);
     bool flip = (eij4 > 0.0f) & (x6 > 0.5f);
     float e = eij4 * (flip ? base + 0.5f : -base);
+    e = e + qq * __builtin_amdgcn_exp2f(r2 * rinv * kdh) * rinv;
     return fminf(e, 1e30f);
 }
 
@@ -237,6 +252,17 @@ __device__ __forceinline__ double wave_sum_dpp(double v)
     return __hiloint2double(hi, lo);
 }
 
+// Squared distance in units of the box between box-scaled positions (x/box):
+// the minimum image of a scaled difference d is d - rint(d).
+__device__ __forceinline__ float dist2_unit(float xj, float yj, float zj, float4 a)
+{
+    float dx = xj - a.x, dy = yj - a.y, dz = zj - a.z;
+    dx -= __builtin_rintf(dx);
+    dy -= __builtin_rintf(dy);
+    dz -= __builtin_rintf(dz);
+    return __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz));
+}
+
 __device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, const DevSys &S)
 {
     const float dx = min_image(xj - a.x, S.box, S.inv_box);
@@ -248,7 +274,8 @@ __device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, c
 // Dynamic LDS of the MC kernel, carved in this order (mc_dynamic_lds sizes it):
 //   draw2:                  StepDraw[DRAW_BATCH]              the draw batch formed ahead
 //   cache_quat:             double q[4*nbody]                 body rotations
-//   always:                 float4 old[mb], new[mb]; int tc[mb] moved-body staging
+//   always:                 float4 old[mb], new[mb], olds[mb], news[mb]; int tc[mb]
+//                           moved-body staging (olds/news: box-scaled copies)
 //   CACHE:                  float x,y,z,q,sig[cl]; int tc[cl]  partner beads [clo, clo+cl)
 //   CACHE && cache_rel:     float rx,ry,rz[cl]                 relative positions
 //   CACHE && cache_bodies:  int start,end[nbody]; float cx,cy,cz[nbody]
@@ -256,7 +283,7 @@ __device__ __forceinline__ float dist2(float xj, float yj, float zj, float4 a, c
 size_t mc_dynamic_lds(const KParams &p)
 {
     const size_t mb = p.lds_mb, cl = p.lds_cl, nb = p.lds_nbody;
-    size_t s = 36 * mb;
+    size_t s = 68 * mb;
     if (p.draw2) s += mc_draw_batch_bytes();
     if (p.cache_quat) s += 32 * nb;
     if (cl) s += 24 * cl;
@@ -321,7 +348,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     lp += p.cache_quat ? 32 * (size_t)p.lds_nbody : 0;
     float4 *s_old = (float4 *)lp;
     float4 *s_new = s_old + p.lds_mb;
-    int *s_tc = (int *)(s_new + p.lds_mb);
+    float4 *s_olds = s_new + p.lds_mb, *s_news = s_olds + p.lds_mb;
+    int *s_tc = (int *)(s_news + p.lds_mb);
     float *s_jx = (float *)(s_tc + p.lds_mb);
     float *s_jy = s_jx + p.lds_cl, *s_jz = s_jy + p.lds_cl, *s_jq = s_jz + p.lds_cl, *s_js = s_jq + p.lds_cl;
     int *s_jtc = (int *)(s_js + p.lds_cl);
@@ -965,9 +993,14 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             float nrx = rx, nry = ry, nrz = rz;
             if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
             const float qb = inc ? s_jq[b - clo] : qs[b];
-            s_old[k] = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
-                           : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
-            s_new[k] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+            const float4 ov = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
+                                  : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
+            const float4 nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+            s_old[k] = ov;
+            s_new[k] = nv;
+            const float ib = S.inv_box;
+            s_olds[k] = make_float4(ov.x * ib, ov.y * ib, ov.z * ib, ov.w * ib);
+            s_news[k] = make_float4(nv.x * ib, nv.y * ib, nv.z * ib, 0.5f * nv.w);
             s_tc[k] = inc ? s_jtc[b - clo] : tcs[b];
         }
         __syncthreads();
@@ -991,7 +1024,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         pick_split<BLOCK>(nm, nj, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
-        float acc = 0.0f, comp = 0.0f;
+        // each FP32 difference en - eo is added in FP64 (one conversion and
+        // one add, fewer instructions than a compensated FP32 sum)
+        double acc = 0.0;
         // two partner beads in registers per pass (jj0 and jj0 + G): each moved
         // bead's staging is read once for both
         for (int jj0 = l; jj0 < nj; jj0 += 2 * G) {
@@ -1029,31 +1064,45 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // against both partners) are formed side by side with no
                 // per-lane branch, four independent chains per wave.  The
                 // Debye-Hueckel part is skipped as a whole when no moved bead
-                // of the wave is charged.
+                // of the wave is charged.  Distances are taken in units of the
+                // box, where the minimum image is d - rint(d): the box-scaled
+                // staging holds x/box and q_i c_dh / box, and the constants
+                // carry the box (sigma^2 / box^2, box k_dh), so every term is
+                // the pair energy of the same formula.
+                const float ib = S.inv_box, ib2 = ib * ib, kdh = ff.k_dh * S.box;
+                float xs[2], ys[2], zs[2], hj[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    xs[u] = xj[u] * ib;
+                    ys[u] = yj[u] * ib;
+                    zs[u] = zj[u] * ib;
+                    hj[u] = 0.5f * sj[u];
+                }
                 const int t0 = tcj[0] & 0xff, t1 = tcj[1] & 0xff;
                 for (int k = k0; k < k1; ++k) {
-                    const float4 o = s_old[k], n = s_new[k];
+                    const float4 o = s_olds[k], n = s_news[k];
                     const float *erow = s_etab + (s_tc[k] & 0xff) * REMC_NTYPES;
                     const float e0 = erow[t0], e1 = erow[t1];
-                    const float h0 = 0.5f * (n.w + sj[0]), h1 = 0.5f * (n.w + sj[1]);
-                    const float ss0 = h0 * h0, ss1 = h1 * h1;
-                    const float ro0 = dist2(xj[0], yj[0], zj[0], o, S), rn0 = dist2(xj[0], yj[0], zj[0], n, S);
-                    const float ro1 = dist2(xj[1], yj[1], zj[1], o, S), rn1 = dist2(xj[1], yj[1], zj[1], n, S);
+                    const float h0 = n.w + hj[0], h1 = n.w + hj[1];
+                    const float ss0 = (h0 * h0) * ib2, ss1 = (h1 * h1) * ib2;
+                    const float ro0 = dist2_unit(xs[0], ys[0], zs[0], o), rn0 = dist2_unit(xs[0], ys[0], zs[0], n);
+                    const float ro1 = dist2_unit(xs[1], ys[1], zs[1], o), rn1 = dist2_unit(xs[1], ys[1], zs[1], n);
                     float eo0, en0, eo1, en1;
                     if (__any(o.w != 0.0f)) {
                         const float qq0 = o.w * qj[0], qq1 = o.w * qj[1];
-                        eo0 = pair_lj_dh<false>(ro0, e0, ss0, qq0, ff);
-                        en0 = pair_lj_dh<false>(rn0, e0, ss0, qq0, ff);
-                        eo1 = pair_lj_dh<false>(ro1, e1, ss1, qq1, ff);
-                        en1 = pair_lj_dh<false>(rn1, e1, ss1, qq1, ff);
+                        eo0 = pair_lj_dh_k(ro0, e0, ss0, qq0, kdh);
+                        en0 = pair_lj_dh_k(rn0, e0, ss0, qq0, kdh);
+                        eo1 = pair_lj_dh_k(ro1, e1, ss1, qq1, kdh);
+                        en1 = pair_lj_dh_k(rn1, e1, ss1, qq1, kdh);
                     } else {
                         eo0 = pair_lj(ro0, e0, ss0);
                         en0 = pair_lj(rn0, e0, ss0);
                         eo1 = pair_lj(ro1, e1, ss1);
                         en1 = pair_lj(rn1, e1, ss1);
                     }
-                    kahan_add(acc, comp, en0 - eo0);
-                    kahan_add(acc, comp, on[1] ? en1 - eo1 : 0.0f);
+                    // the two partners' differences are added in FP32 (one
+                    // rounding, as each difference has), then in FP64
+                    acc += (double)((en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
                 }
                 continue;
             }
@@ -1075,11 +1124,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         eo = pair_lj_dh(dist2(xj[u], yj[u], zj[u], o, S), eij, sij2, qq, ff);
                         en = pair_lj_dh(dist2(xj[u], yj[u], zj[u], n, S), eij, sij2, qq, ff);
                     }
-                    kahan_add(acc, comp, en - eo);
+                    acc += (double)(en - eo);
                 }
             }
         }
-        const double v = wave_sum_dpp((double)acc - (double)comp);
+        const double v = wave_sum_dpp(acc);
         if (lane == 0) s_red[wave] = v;
         __syncthreads();
         double dE = 0.0;
