@@ -113,6 +113,15 @@ __device__ __forceinline__ float pair_crowder(float r2, const DevFF &ff)
     return r2 < ff.crow_cut2 ? fminf(ff.crow_k * rc3, 1e30f) : 0.0f;
 }
 
+// pair_crowder with the squared cutoff and the constant given (in the length
+// unit of r2).
+__device__ __forceinline__ float pair_crowder_u(float r2, float cut2, float ck)
+{
+    const float rc = __builtin_amdgcn_rcpf(r2);
+    const float rc3 = rc * rc * rc;
+    return r2 < cut2 ? fminf(ck * rc3, 1e30f) : 0.0f;
+}
+
 // Squared screening radius of a one-bead crowder move from o to n: a partner
 // bead farther than the cutoff plus the move's length (+0.01 A for the FP32
 // rounding of the distances) from o is beyond the cutoff of both positions,
@@ -211,6 +220,38 @@ __device__ __forceinline__ void pick_split(int nm, int nj, int &G, int &kper)
             kper = kp;
         }
     }
+}
+
+// The split of the MC kernel's pair loop, which holds two partners per lane
+// per pass and runs all four terms of a pass whether or not the second
+// partner exists.  Lane i of each wave prices g = i + 1 groups of
+// G = BLOCK / g lanes: ceil(ceil(nj / G) / 2) passes, each over
+// kper = ceil(nm / g) moved beads plus a quarter for the pass's partner loads,
+// 5% more where group boundaries fall inside waves (a wave then spans several
+// moved beads and takes the charged-pair branch more often); the cheapest,
+// lowest g on ties, is taken by every wave alike.
+template <int BLOCK>
+__device__ __forceinline__ void pick_split_pairs(int nm, int nj, int lane, int &G, int &kper)
+{
+    if (nj <= 0) {
+        G = 1;
+        kper = nm;
+        return;
+    }
+    const int g = lane + 1;
+    const int Gl = div_floor_small(BLOCK, g);
+    unsigned int key = 0xFFFFFFFFu;
+    if (Gl >= 1) {
+        const int passes = (div_ceil_small(nj, Gl) + 1) >> 1;
+        const int kp = div_ceil_small(nm, g);
+        const long long cost = (long long)passes * (kp * ((Gl & 63) ? 105 : 100) + 25);
+        key = (unsigned int)min(cost, (long long)(1 << 25) - 1) << 6 | (unsigned int)lane;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) key = min(key, (unsigned int)__shfl_xor((int)key, off, 64));
+    const int gb = (int)(key & 63u) + 1;
+    G = div_floor_small(BLOCK, gb);
+    kper = div_ceil_small(nm, gb);
 }
 
 // Draws of one MC step and the move parameters that depend on them only
@@ -1021,7 +1062,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         const int ov = even ? nm : max(0, min(je, m1) - ovs);
         const int nj = even ? je - js : (je - js) - ov;
         int G, kper;
-        pick_split<BLOCK>(nm, nj, G, kper);
+        pick_split_pairs<BLOCK>(nm, nj, lane, G, kper);
+        G = __builtin_amdgcn_readfirstlane(G);
+        kper = __builtin_amdgcn_readfirstlane(kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         // each FP32 difference en - eo is added in FP64 (one conversion and
@@ -1102,6 +1145,30 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     }
                     // the two partners' differences are added in FP32 (one
                     // rounding, as each difference has), then in FP64
+                    acc += (double)((en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
+                }
+                continue;
+            }
+            if (__all(cr0 && (!on[1] || cr1))) {
+                // every active lane's pairs are crowder pairs (the moved body or
+                // both partners are crowders): the same four-chain form with
+                // the crowder term only, in box units (cutoff^2 / box^2,
+                // C sigma_c^6 / box^6)
+                const float ib = S.inv_box, ib2 = ib * ib;
+                const float cut2 = ff.crow_cut2 * ib2, ck = ff.crow_k * (ib2 * ib2 * ib2);
+                float xs[2], ys[2], zs[2];
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    xs[u] = xj[u] * ib;
+                    ys[u] = yj[u] * ib;
+                    zs[u] = zj[u] * ib;
+                }
+                for (int k = k0; k < k1; ++k) {
+                    const float4 o = s_olds[k], n = s_news[k];
+                    const float eo0 = pair_crowder_u(dist2_unit(xs[0], ys[0], zs[0], o), cut2, ck);
+                    const float en0 = pair_crowder_u(dist2_unit(xs[0], ys[0], zs[0], n), cut2, ck);
+                    const float eo1 = pair_crowder_u(dist2_unit(xs[1], ys[1], zs[1], o), cut2, ck);
+                    const float en1 = pair_crowder_u(dist2_unit(xs[1], ys[1], zs[1], n), cut2, ck);
                     acc += (double)((en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
                 }
                 continue;
