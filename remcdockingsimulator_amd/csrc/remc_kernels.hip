@@ -37,6 +37,15 @@
 
 namespace remc {
 
+// A load from LDS through a pointer the compiler cannot place (it derives
+// from the dynamic LDS block, or meets a global pointer in a select, which
+// becomes a flat load): the address-space cast makes it an LDS read.
+template <typename T>
+__device__ __forceinline__ T lds_ld(const T *ptr)
+{
+    return *(const __attribute__((address_space(3))) T *)ptr;
+}
+
 __device__ __forceinline__ float min_image(float d, float box, float inv_box)
 {
     return __builtin_fmaf(-box, __builtin_rintf(d * inv_box), d);
@@ -476,6 +485,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     int dbuf = 0;        // the buffer of the current batch
     bool nb_ok = false;  // the other buffer holds the batch from nb0
     int nb0 = 0;
+    int pre_w = 0;       // slots of the next window, staged ahead (0: none)
     int it = 0;
     while (it < nsteps) {
         // The thread index is re-read opaquely in every step, so that the
@@ -514,8 +524,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     sd.body = dd.body;
                     sd.rotate = dd.rotate;
                     const int gb = S.body_off + dd.body;
-                    sd.m0 = cache_bodies ? s_bs[dd.body] : p.bstart[gb];
-                    sd.nm = (cache_bodies ? s_be[dd.body] : p.bend[gb]) - sd.m0;
+                    sd.m0 = cache_bodies ? lds_ld(s_bs + dd.body) : p.bstart[gb];
+                    sd.nm = (cache_bodies ? lds_ld(s_be + dd.body) : p.bend[gb]) - sd.m0;
                     if (SPEC) el = sd.nm <= SPEC_MAX;
                 }
                 if (SPEC) {
@@ -568,6 +578,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 nb_ok = false;
                 if (SPEC) elig = s_elig[dbuf];
             } else {
+                pre_w = 0; // a window staged ahead came from the old batch
                 form_draws(dbuf ? s_draw1 : s_draw0, dbuf, it, step, wave);
                 b0 = it;
                 have = min(DRAW_BATCH, nsteps - it);
@@ -578,6 +589,103 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             }
         }
         StepDraw *const s_draw = dbuf ? s_draw1 : s_draw0;
+        // Staging of a speculative window's slots [0, wn) from step off_ of
+        // the draw batch sd, one thread st per slot bead (slot st / SPEC_MAX,
+        // bead st % SPEC_MAX): the moved beads' old and new positions, the
+        // slot's static and chained flags, and its new rotation or centre.  It
+        // reads only the slots' own bodies.
+        auto stage_window = [&](const int st, const int off_, const int wn, const StepDraw *sd) {
+            const int k = st / SPEC_MAX, i = st % SPEC_MAX;
+            if (k < wn) {
+                const StepDraw &Dk = sd[off_ + k];
+                const int bk = Dk.body;
+                const int m0k = Dk.m0, nmk = Dk.nm;
+                // One earlier slot j of the window moves the same body: slot k
+                // is chained, staged from the state j's proposal leaves, and
+                // stands if j is accepted.  Two or more: staged from the
+                // window's start, it stands only if all of them are rejected.
+                const int pj = Dk.prev1 - off_;
+                const bool chained = pj >= 0 && Dk.prev2 < off_;
+                if (i == 0) {
+                    s_wchain[k] = chained ? 1u << pj : 0u;
+                    s_wnext[k] = -1;
+                }
+                if (i < nmk) {
+                    // the moved beads come from the partner cache when it holds
+                    // them (a member's slice may not), else from HBM
+                    const int b = m0k + i;
+                    const bool inc = b >= clo && b < chi;
+                    // typed loads: LDS reads where the data is cached (a
+                    // select between the two pointers would be a flat load)
+                    float rx, ry, rz, cx, cy, cz, ox, oy, oz;
+                    if (cache_rel) {
+                        rx = lds_ld(s_rx + b);
+                        ry = lds_ld(s_ry + b);
+                        rz = lds_ld(s_rz + b);
+                    } else {
+                        rx = grx[b];
+                        ry = gry[b];
+                        rz = grz[b];
+                    }
+                    if (cache_bodies) {
+                        cx = lds_ld(s_cx + bk);
+                        cy = lds_ld(s_cy + bk);
+                        cz = lds_ld(s_cz + bk);
+                    } else {
+                        cx = gcx[bk];
+                        cy = gcy[bk];
+                        cz = gcz[bk];
+                    }
+                    if (inc) {
+                        ox = lds_ld(s_jx + (b - clo));
+                        oy = lds_ld(s_jy + (b - clo));
+                        oz = lds_ld(s_jz + (b - clo));
+                    } else {
+                        ox = gx[b];
+                        oy = gy[b];
+                        oz = gz[b];
+                    }
+                    if (chained) {
+                        // slot j's proposal, formed as slot j's own staging forms it
+                        const StepDraw &Dj = sd[off_ + pj];
+                        if (Dj.rotate) remc_qrotate(Dj.coef, rx, ry, rz, &rx, &ry, &rz);
+                        else remc_wrap_centre(cx, cy, cz, Dj.t[0], Dj.t[1], Dj.t[2], S.box, &cx, &cy, &cz);
+                        ox = rx + cx;
+                        oy = ry + cy;
+                        oz = rz + cz;
+                    }
+                    float nrx = rx, nry = ry, nrz = rz, ncx = cx, ncy = cy, ncz = cz;
+                    if (Dk.rotate) remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
+                    else remc_wrap_centre(cx, cy, cz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
+                    const float qb = inc ? lds_ld(s_jq + (b - clo)) : qs[b];
+                    const float sb = inc ? lds_ld(s_js + (b - clo)) : sgs[b];
+                    s_wold[st] = make_float4(ox, oy, oz, qb * ff.c_dh);
+                    s_wnew[st] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sb);
+                    // a static slot: one bead whose move leaves it where it is
+                    if (i == 0)
+                        s_wstat[k] = nmk == 1 && ox == nrx + ncx && oy == nry + ncy && oz == nrz + ncz ? 1 : 0;
+                    s_wrel[st] = make_float4(nrx, nry, nrz, qb);
+                    s_wtc[st] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
+                    if (i == 0) {
+                        if (Dk.rotate) {
+                            double qo[4], qn[4];
+                            if (cache_quat)
+                                for (int t = 0; t < 4; ++t) qo[t] = lds_ld(s_q + 4 * bk + t);
+                            else
+                                for (int t = 0; t < 4; ++t) qo[t] = gq[4 * bk + t];
+                            if (chained && sd[off_ + pj].rotate) {
+                                remc_qmul(sd[off_ + pj].q, qo, qn);
+                                for (int t = 0; t < 4; ++t) qo[t] = qn[t];
+                            }
+                            remc_qmul(Dk.q, qo, qn);
+                            for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
+                        } else {
+                            s_wcen[k] = make_float4(ncx, ncy, ncz, 0.0f);
+                        }
+                    }
+                }
+            }
+        };
 
         int w = 0;
         if (SPEC) {
@@ -585,6 +693,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             w = run == 0ull ? 64 : (int)__builtin_ctzll(run);
             w = __builtin_amdgcn_readfirstlane(min(w, WK)); // the same in every lane
         }
+        // a window staged ahead, during the exchange of the large move before it
+        const bool pre_ok = SPEC && pre_w > 0 && pre_w == w;
+        pre_w = 0;
         if (SPEC && w > 0) {
             // ---------------- speculative window of w small moves ----------------
             // staging: thread t -> slot k = t / SPEC_MAX, bead i = t % SPEC_MAX
@@ -596,71 +707,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 form_draws(dbuf ? s_draw0 : s_draw1, dbuf ^ 1, nb0, step - (unsigned long long)(it - nb0), wave - NW / 2);
                 nb_ok = true;
             }
-            if (tid < WK * SPEC_MAX) {
-                const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
-                if (k < w) {
-                    const StepDraw &Dk = s_draw[off + k];
-                    const int bk = Dk.body;
-                    const int m0k = Dk.m0, nmk = Dk.nm;
-                    // One earlier slot j of the window moves the same body: slot k
-                    // is chained, staged from the state j's proposal leaves, and
-                    // stands if j is accepted.  Two or more: staged from the
-                    // window's start, it stands only if all of them are rejected.
-                    const int pj = Dk.prev1 - off;
-                    const bool chained = pj >= 0 && Dk.prev2 < off;
-                    if (i == 0) {
-                        s_wchain[k] = chained ? 1u << pj : 0u;
-                        s_wnext[k] = -1;
-                    }
-                    if (i < nmk) {
-                        // the moved beads come from the partner cache when it holds
-                        // them (a member's slice may not), else from HBM
-                        const int b = m0k + i;
-                        const bool inc = b >= clo && b < chi;
-                        float rx = cache_rel ? s_rx[b] : grx[b];
-                        float ry = cache_rel ? s_ry[b] : gry[b];
-                        float rz = cache_rel ? s_rz[b] : grz[b];
-                        float cx = cache_bodies ? s_cx[bk] : gcx[bk], cy = cache_bodies ? s_cy[bk] : gcy[bk],
-                              cz = cache_bodies ? s_cz[bk] : gcz[bk];
-                        float ox = inc ? s_jx[b - clo] : gx[b], oy = inc ? s_jy[b - clo] : gy[b],
-                              oz = inc ? s_jz[b - clo] : gz[b];
-                        if (chained) {
-                            // slot j's proposal, formed as slot j's own staging forms it
-                            const StepDraw &Dj = s_draw[off + pj];
-                            if (Dj.rotate) remc_qrotate(Dj.coef, rx, ry, rz, &rx, &ry, &rz);
-                            else remc_wrap_centre(cx, cy, cz, Dj.t[0], Dj.t[1], Dj.t[2], S.box, &cx, &cy, &cz);
-                            ox = rx + cx;
-                            oy = ry + cy;
-                            oz = rz + cz;
-                        }
-                        float nrx = rx, nry = ry, nrz = rz, ncx = cx, ncy = cy, ncz = cz;
-                        if (Dk.rotate) remc_qrotate(Dk.coef, rx, ry, rz, &nrx, &nry, &nrz);
-                        else remc_wrap_centre(cx, cy, cz, Dk.t[0], Dk.t[1], Dk.t[2], S.box, &ncx, &ncy, &ncz);
-                        const float qb = inc ? s_jq[b - clo] : qs[b];
-                        s_wold[tid] = make_float4(ox, oy, oz, qb * ff.c_dh);
-                        s_wnew[tid] = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
-                        // a static slot: one bead whose move leaves it where it is
-                        if (i == 0)
-                            s_wstat[k] = nmk == 1 && ox == nrx + ncx && oy == nry + ncy && oz == nrz + ncz ? 1 : 0;
-                        s_wrel[tid] = make_float4(nrx, nry, nrz, qb);
-                        s_wtc[tid] = inc ? s_jtc[b - clo] : tcs[b];
-                        if (i == 0) {
-                            if (Dk.rotate) {
-                                double qo[4], qn[4];
-                                for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * bk + t] : gq[4 * bk + t];
-                                if (chained && s_draw[off + pj].rotate) {
-                                    remc_qmul(s_draw[off + pj].q, qo, qn);
-                                    for (int t = 0; t < 4; ++t) qo[t] = qn[t];
-                                }
-                                remc_qmul(Dk.q, qo, qn);
-                                for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
-                            } else {
-                                s_wcen[k] = make_float4(ncx, ncy, ncz, 0.0f);
-                            }
-                        }
-                    }
-                }
-            }
+            if (!pre_ok && tid < WK * SPEC_MAX) stage_window(tid, off, w, s_draw);
             __syncthreads();
 
             // dE0 of slots k = wave and wave + NW: the moved beads against every
@@ -1007,11 +1054,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         int m0, m1;
         float cx, cy, cz;
         if (cache_bodies) {
-            m0 = s_bs[body];
-            m1 = s_be[body];
-            cx = s_cx[body];
-            cy = s_cy[body];
-            cz = s_cz[body];
+            m0 = lds_ld(s_bs + body);
+            m1 = lds_ld(s_be + body);
+            cx = lds_ld(s_cx + body);
+            cy = lds_ld(s_cy + body);
+            cz = lds_ld(s_cz + body);
         } else {
             const int gb = S.body_off + body;
             m0 = p.bstart[gb];
@@ -1028,21 +1075,33 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         for (int k = tid; k < nm; k += BLOCK) {
             const int b = m0 + k;
             const bool inc = CACHE && b >= clo && b < chi;
-            const float rx = cache_rel ? s_rx[b] : grx[b];
-            const float ry = cache_rel ? s_ry[b] : gry[b];
-            const float rz = cache_rel ? s_rz[b] : grz[b];
+            float rx, ry, rz;
+            if (cache_rel) {
+                rx = lds_ld(s_rx + b);
+                ry = lds_ld(s_ry + b);
+                rz = lds_ld(s_rz + b);
+            } else {
+                rx = grx[b];
+                ry = gry[b];
+                rz = grz[b];
+            }
             float nrx = rx, nry = ry, nrz = rz;
             if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
-            const float qb = inc ? s_jq[b - clo] : qs[b];
-            const float4 ov = inc ? make_float4(s_jx[b - clo], s_jy[b - clo], s_jz[b - clo], qb * ff.c_dh)
-                                  : make_float4(gx[b], gy[b], gz[b], qb * ff.c_dh);
-            const float4 nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, inc ? s_js[b - clo] : sgs[b]);
+            float4 ov, nv;
+            if (inc) {
+                const int kb = b - clo;
+                ov = make_float4(lds_ld(s_jx + kb), lds_ld(s_jy + kb), lds_ld(s_jz + kb), lds_ld(s_jq + kb) * ff.c_dh);
+                nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, lds_ld(s_js + kb));
+            } else {
+                ov = make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
+                nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
+            }
             s_old[k] = ov;
             s_new[k] = nv;
             const float ib = S.inv_box;
             s_olds[k] = make_float4(ov.x * ib, ov.y * ib, ov.z * ib, ov.w * ib);
             s_news[k] = make_float4(nv.x * ib, nv.y * ib, nv.z * ib, 0.5f * nv.w);
-            s_tc[k] = inc ? s_jtc[b - clo] : tcs[b];
+            s_tc[k] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
         }
         __syncthreads();
 
@@ -1203,6 +1262,35 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 #pragma unroll
             for (int w2 = 0; w2 < NW; ++w2) dE += s_red[w2];
         } else {
+            // When the next step starts a window in this draw batch (no refill
+            // due), waves 1.. stage it while wave 0 waits for the members'
+            // partials: the staging reads only the window's bodies, which this
+            // step's move does not change.
+            // The refill rule of the next step is applied here as the step
+            // applies it; a refill that switches to the batch formed ahead
+            // leaves the window's draws known.
+            int nw = 0, noff = off + 1;
+            const StepDraw *nsd = s_draw;
+            if (SPEC && it + 1 < nsteps) {
+                bool refill1 = noff >= have;
+                if (!refill1 && noff + WK > have && b0 + have < nsteps) {
+                    const unsigned long long rest = (1ull << (have - noff)) - 1ull; // have - noff < WK <= 32
+                    refill1 = ((~(elig >> noff)) & rest) == 0ull;
+                }
+                unsigned long long el1 = elig;
+                if (refill1 && nb_ok && it + 1 >= nb0) {
+                    noff = it + 1 - nb0;
+                    el1 = s_elig[dbuf ^ 1];
+                    nsd = dbuf ? s_draw0 : s_draw1;
+                    refill1 = false;
+                }
+                if (!refill1) {
+                    const unsigned long long run = ~(el1 >> noff);
+                    nw = run == 0ull ? 64 : (int)__builtin_ctzll(run);
+                    nw = __builtin_amdgcn_readfirstlane(min(nw, WK));
+                }
+            }
+            if (wave != 0 && nw > 0 && tid - 64 < WK * SPEC_MAX) stage_window(tid - 64, noff, nw, nsd);
             if (wave == 0) {
                 double part = 0.0;
 #pragma unroll
@@ -1248,6 +1336,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (s_abort) break;
             dE = s_dE;
             ++n_x;
+            pre_w = nw;
         }
 
         const int decision = remc_metropolis_l(dE, beta, D.nlu);
@@ -1264,9 +1353,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     s_jz[b - clo] = n.z;
                 }
                 if (rotate) {
-                    const float rx = cache_rel ? s_rx[b] : grx[b];
-                    const float ry = cache_rel ? s_ry[b] : gry[b];
-                    const float rz = cache_rel ? s_rz[b] : grz[b];
+                    float rx, ry, rz;
+                    if (cache_rel) {
+                        rx = lds_ld(s_rx + b);
+                        ry = lds_ld(s_ry + b);
+                        rz = lds_ld(s_rz + b);
+                    } else {
+                        rx = grx[b];
+                        ry = gry[b];
+                        rz = grz[b];
+                    }
                     float nrx, nry, nrz;
                     remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
                     grx[b] = nrx;
@@ -1282,7 +1378,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (tid == 0) {
                 if (rotate) {
                     double qo[4], qn[4];
-                    for (int t = 0; t < 4; ++t) qo[t] = cache_quat ? s_q[4 * body + t] : gq[4 * body + t];
+                    if (cache_quat)
+                        for (int t = 0; t < 4; ++t) qo[t] = lds_ld(s_q + 4 * body + t);
+                    else
+                        for (int t = 0; t < 4; ++t) qo[t] = gq[4 * body + t];
                     remc_qmul(D.q, qo, qn);
                     for (int t = 0; t < 4; ++t) {
                         gq[4 * body + t] = qn[t];
