@@ -486,6 +486,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     bool nb_ok = false;  // the other buffer holds the batch from nb0
     int nb0 = 0;
     int pre_w = 0;       // slots of the next window, staged ahead (0: none)
+    int pre_it = -1;     // the large move staged ahead (launch step index; -1: none)
     int it = 0;
     while (it < nsteps) {
         // The thread index is re-read opaquely in every step, so that the
@@ -589,6 +590,60 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             }
         }
         StepDraw *const s_draw = dbuf ? s_draw1 : s_draw0;
+        // Staging of a large move (draws Dm) into s_old / s_new, their
+        // box-scaled copies and s_tc, by threads t0, t0 + stride, ... over its
+        // beads.  It reads only the moved body.
+        auto stage_move = [&](const int t0, const int stride, const StepDraw &Dm) {
+            const int bd = Dm.body;
+            int mb0, mb1;
+            float bcx, bcy, bcz;
+            if (cache_bodies) {
+                mb0 = lds_ld(s_bs + bd);
+                mb1 = lds_ld(s_be + bd);
+                bcx = lds_ld(s_cx + bd);
+                bcy = lds_ld(s_cy + bd);
+                bcz = lds_ld(s_cz + bd);
+            } else {
+                mb0 = p.bstart[S.body_off + bd];
+                mb1 = p.bend[S.body_off + bd];
+                bcx = gcx[bd];
+                bcy = gcy[bd];
+                bcz = gcz[bd];
+            }
+            float nbx = bcx, nby = bcy, nbz = bcz;
+            if (!Dm.rotate) remc_wrap_centre(bcx, bcy, bcz, Dm.t[0], Dm.t[1], Dm.t[2], S.box, &nbx, &nby, &nbz);
+            for (int k = t0; k < mb1 - mb0; k += stride) {
+                const int b = mb0 + k;
+                const bool inc = CACHE && b >= clo && b < chi;
+                float rx, ry, rz;
+                if (cache_rel) {
+                    rx = lds_ld(s_rx + b);
+                    ry = lds_ld(s_ry + b);
+                    rz = lds_ld(s_rz + b);
+                } else {
+                    rx = grx[b];
+                    ry = gry[b];
+                    rz = grz[b];
+                }
+                float nrx = rx, nry = ry, nrz = rz;
+                if (Dm.rotate) remc_qrotate(Dm.coef, rx, ry, rz, &nrx, &nry, &nrz);
+                float4 ov, nv;
+                if (inc) {
+                    const int kb = b - clo;
+                    ov = make_float4(lds_ld(s_jx + kb), lds_ld(s_jy + kb), lds_ld(s_jz + kb), lds_ld(s_jq + kb) * ff.c_dh);
+                    nv = make_float4(nrx + nbx, nry + nby, nrz + nbz, lds_ld(s_js + kb));
+                } else {
+                    ov = make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
+                    nv = make_float4(nrx + nbx, nry + nby, nrz + nbz, sgs[b]);
+                }
+                s_old[k] = ov;
+                s_new[k] = nv;
+                const float ib = S.inv_box;
+                s_olds[k] = make_float4(ov.x * ib, ov.y * ib, ov.z * ib, ov.w * ib);
+                s_news[k] = make_float4(nv.x * ib, nv.y * ib, nv.z * ib, 0.5f * nv.w);
+                s_tc[k] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
+            }
+        };
         // Staging of a speculative window's slots [0, wn) from step off_ of
         // the draw batch sd, one thread st per slot bead (slot st / SPEC_MAX,
         // bead st % SPEC_MAX): the moved beads' old and new positions, the
@@ -872,6 +927,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             }
             __syncthreads();
 
+            // The window's run of small moves ends at a large move of this
+            // batch: waves 1.. stage it while wave 0 gathers the partials and
+            // decides.  It reads only the large move's body, which no slot of
+            // the window moves, so the staging holds whatever the decisions.
+            const int nxo = off + w;
+            const bool nx_big = w < WK && nxo < have && it + w < nsteps && ((elig >> nxo) & 1ull) == 0ull;
+            if (wave != 0 && nx_big) stage_move(tid - 64, BLOCK - 64, s_draw[nxo]);
+            if (nx_big) pre_it = it + w;
+
             // decisions in step order (wave 0); lane l carries slot l's dE
             if (wave == 0) {
                 // the slot's uniform and masks, read before the wait for the partials
@@ -1049,6 +1113,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 
         // ---------------------------- one step ----------------------------
         const StepDraw &D = s_draw[off];
+        const bool pre_mv = SPEC && pre_it == it; // staged during the window before it
+        pre_it = -1;
         const int body = D.body;
         const bool rotate = D.rotate != 0;
         int m0, m1;
@@ -1072,37 +1138,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                                                              : p.bcrow[S.body_off + body] != 0;
         float ncx = cx, ncy = cy, ncz = cz;
         if (!rotate) remc_wrap_centre(cx, cy, cz, D.t[0], D.t[1], D.t[2], S.box, &ncx, &ncy, &ncz);
-        for (int k = tid; k < nm; k += BLOCK) {
-            const int b = m0 + k;
-            const bool inc = CACHE && b >= clo && b < chi;
-            float rx, ry, rz;
-            if (cache_rel) {
-                rx = lds_ld(s_rx + b);
-                ry = lds_ld(s_ry + b);
-                rz = lds_ld(s_rz + b);
-            } else {
-                rx = grx[b];
-                ry = gry[b];
-                rz = grz[b];
-            }
-            float nrx = rx, nry = ry, nrz = rz;
-            if (rotate) remc_qrotate(D.coef, rx, ry, rz, &nrx, &nry, &nrz);
-            float4 ov, nv;
-            if (inc) {
-                const int kb = b - clo;
-                ov = make_float4(lds_ld(s_jx + kb), lds_ld(s_jy + kb), lds_ld(s_jz + kb), lds_ld(s_jq + kb) * ff.c_dh);
-                nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, lds_ld(s_js + kb));
-            } else {
-                ov = make_float4(gx[b], gy[b], gz[b], qs[b] * ff.c_dh);
-                nv = make_float4(nrx + ncx, nry + ncy, nrz + ncz, sgs[b]);
-            }
-            s_old[k] = ov;
-            s_new[k] = nv;
-            const float ib = S.inv_box;
-            s_olds[k] = make_float4(ov.x * ib, ov.y * ib, ov.z * ib, ov.w * ib);
-            s_news[k] = make_float4(nv.x * ib, nv.y * ib, nv.z * ib, 0.5f * nv.w);
-            s_tc[k] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
-        }
+        if (!pre_mv) stage_move(tid, BLOCK, D);
         __syncthreads();
 
         // Moves with few pair terms are evaluated whole by every member (no
