@@ -638,9 +638,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 }
                 s_old[k] = ov;
                 s_new[k] = nv;
+                // box-scaled about the body's old centre: the beads near it, which
+                // give the steep contact terms, keep every bit of their offsets
                 const float ib = S.inv_box;
-                s_olds[k] = make_float4(ov.x * ib, ov.y * ib, ov.z * ib, ov.w * ib);
-                s_news[k] = make_float4(nv.x * ib, nv.y * ib, nv.z * ib, 0.5f * nv.w);
+                s_olds[k] = make_float4((ov.x - bcx) * ib, (ov.y - bcy) * ib, (ov.z - bcz) * ib, ov.w * ib);
+                s_news[k] = make_float4((nv.x - bcx) * ib, (nv.y - bcy) * ib, (nv.z - bcz) * ib, 0.5f * nv.w);
                 s_tc[k] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
             }
         };
@@ -1204,16 +1206,17 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 // Debye-Hueckel part is skipped as a whole when no moved bead
                 // of the wave is charged.  Distances are taken in units of the
                 // box, where the minimum image is d - rint(d): the box-scaled
-                // staging holds x/box and q_i c_dh / box, and the constants
-                // carry the box (sigma^2 / box^2, box k_dh), so every term is
-                // the pair energy of the same formula.
+                // staging holds (x - c)/box about the moved body's old centre c
+                // and q_i c_dh / box, the partners are scaled about c alike, and
+                // the constants carry the box (sigma^2 / box^2, box k_dh), so
+                // every term is the pair energy of the same formula.
                 const float ib = S.inv_box, ib2 = ib * ib, kdh = ff.k_dh * S.box;
                 float xs[2], ys[2], zs[2], hj[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    xs[u] = xj[u] * ib;
-                    ys[u] = yj[u] * ib;
-                    zs[u] = zj[u] * ib;
+                    xs[u] = (xj[u] - cx) * ib;
+                    ys[u] = (yj[u] - cy) * ib;
+                    zs[u] = (zj[u] - cz) * ib;
                     hj[u] = 0.5f * sj[u];
                 }
                 const int t0 = tcj[0] & 0xff, t1 = tcj[1] & 0xff;
@@ -1254,9 +1257,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                 float xs[2], ys[2], zs[2];
 #pragma unroll
                 for (int u = 0; u < 2; ++u) {
-                    xs[u] = xj[u] * ib;
-                    ys[u] = yj[u] * ib;
-                    zs[u] = zj[u] * ib;
+                    xs[u] = (xj[u] - cx) * ib;
+                    ys[u] = (yj[u] - cy) * ib;
+                    zs[u] = (zj[u] - cz) * ib;
                 }
                 for (int k = k0; k < k1; ++k) {
                     const float4 o = s_olds[k], n = s_news[k];
