@@ -2108,7 +2108,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         pick_split<BLOCK>(nm, nj, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
+        // Each lane sums its pairs' force and torque in FP32 (a few to a few
+        // hundred terms: about 1e-6 relative, far inside what the proposal and
+        // the proposal ratio need) and the lanes' sums are added in FP64.
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
+        float f6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int jj = l; jj < nj; jj += G) {
             const int j = js + jj + (js + jj >= ovs ? ov : 0);
             const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
@@ -2124,13 +2128,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 float e, g2;
                 pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow,
                         s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, o.w * qj, inv_xi, ff, e, g2);
-                const double fx = -(double)(g2 * dx), fy = -(double)(g2 * dy), fz = -(double)(g2 * dz);
-                v[0] += fx; v[1] += fy; v[2] += fz;
-                v[3] += (double)orl.y * fz - (double)orl.z * fy;
-                v[4] += (double)orl.z * fx - (double)orl.x * fz;
-                v[5] += (double)orl.x * fy - (double)orl.y * fx;
+                const float fx = -(g2 * dx), fy = -(g2 * dy), fz = -(g2 * dz);
+                f6[0] += fx;
+                f6[1] += fy;
+                f6[2] += fz;
+                f6[3] += orl.y * fz - orl.z * fy;
+                f6[4] += orl.z * fx - orl.x * fz;
+                f6[5] += orl.x * fy - orl.y * fx;
             }
         }
+        for (int k = 0; k < 6; ++k) v[k] = (double)f6[k];
         block_sum(v, 6);
         if (split) {
             exchange(s_ft, 6, (2ull * step) & 0x7FFFFull);
@@ -2184,7 +2191,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
 
         // pass B: dE and the force and torque at the new position (same split)
         float acc = 0.0f, comp = 0.0f;
-        for (int k = 0; k < 6; ++k) v[k] = 0.0;
+        for (int k = 0; k < 6; ++k) f6[k] = 0.f;
         for (int jj = l; jj < nj; jj += G) {
             const int j = js + jj + (js + jj >= ovs ? ov : 0);
             const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
@@ -2210,13 +2217,16 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
                 pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi, ff,
                         en, g2);
                 kahan_add(acc, comp, en - eo);
-                const double fx = -(double)(g2 * dx), fy = -(double)(g2 * dy), fz = -(double)(g2 * dz);
-                v[0] += fx; v[1] += fy; v[2] += fz;
-                v[3] += (double)nrl.y * fz - (double)nrl.z * fy;
-                v[4] += (double)nrl.z * fx - (double)nrl.x * fz;
-                v[5] += (double)nrl.x * fy - (double)nrl.y * fx;
+                const float fx = -(g2 * dx), fy = -(g2 * dy), fz = -(g2 * dz);
+                f6[0] += fx;
+                f6[1] += fy;
+                f6[2] += fz;
+                f6[3] += nrl.y * fz - nrl.z * fy;
+                f6[4] += nrl.z * fx - nrl.x * fz;
+                f6[5] += nrl.x * fy - nrl.y * fx;
             }
         }
+        for (int k = 0; k < 6; ++k) v[k] = (double)f6[k];
         v[6] = (double)acc - (double)comp;
         block_sum(v, 7);
         if (split) {
