@@ -124,6 +124,14 @@ REMC_HD float remc_fmodf(float x, float L)
     float ax = fabsf(x);
     if (!(ax >= L)) return x;           // also NaN
     if (!(ax <= 3.4028234663852886e38f)) return x; // inf: no finite remainder, leave it
+    if (ax < 4.0f * L) {
+        // the wraps that occur (L <= |x| < 4L): |x| - L, |x| - 2L and, from
+        // [L, 2L), that minus L are exact (Sterbenz), the same numbers the
+        // subtraction loop below forms, without its branches
+        const float r2 = ax - 2.0f * L;
+        const float r = ax < 2.0f * L ? ax - L : (r2 < L ? r2 : r2 - L);
+        return x < 0.0f ? -r : r;
+    }
     float r = ax;
     while (r >= L) {
         float d = L;
