@@ -1164,9 +1164,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         kper = __builtin_amdgcn_readfirstlane(kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
-        // each FP32 difference en - eo is added in FP64 (one conversion and
-        // one add, fewer instructions than a compensated FP32 sum)
-        double acc = 0.0;
+        // compensated (Kahan) FP32 sum of the differences en - eo per thread
+        float acc = 0.0f, comp = 0.0f;
         // two partner beads in registers per pass (jj0 and jj0 + G): each moved
         // bead's staging is read once for both
         for (int jj0 = l; jj0 < nj; jj0 += 2 * G) {
@@ -1241,9 +1240,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         eo1 = pair_lj(ro1, e1, ss1);
                         en1 = pair_lj(rn1, e1, ss1);
                     }
-                    // the two partners' differences are added in FP32 (one
-                    // rounding, as each difference has), then in FP64
-                    acc += (double)((en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
+                    // the two partners' differences are added (one rounding, as
+                    // each difference has), then to the compensated sum
+                    kahan_add(acc, comp, (en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
                 }
                 continue;
             }
@@ -1267,7 +1266,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     const float en0 = pair_crowder_u(dist2_unit(xs[0], ys[0], zs[0], n), cut2, ck);
                     const float eo1 = pair_crowder_u(dist2_unit(xs[1], ys[1], zs[1], o), cut2, ck);
                     const float en1 = pair_crowder_u(dist2_unit(xs[1], ys[1], zs[1], n), cut2, ck);
-                    acc += (double)((en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
+                    kahan_add(acc, comp, (en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
                 }
                 continue;
             }
@@ -1289,11 +1288,11 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         eo = pair_lj_dh(dist2(xj[u], yj[u], zj[u], o, S), eij, sij2, qq, ff);
                         en = pair_lj_dh(dist2(xj[u], yj[u], zj[u], n, S), eij, sij2, qq, ff);
                     }
-                    acc += (double)(en - eo);
+                    kahan_add(acc, comp, en - eo);
                 }
             }
         }
-        const double v = wave_sum_dpp(acc);
+        const double v = wave_sum_dpp((double)acc - (double)comp);
         if (lane == 0) s_red[wave] = v;
         __syncthreads();
         double dE = 0.0;
@@ -2109,8 +2108,9 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         // Each lane sums its pairs' force and torque in FP32 (a few to a few
-        // hundred terms: about 1e-6 relative, far inside what the proposal and
-        // the proposal ratio need) and the lanes' sums are added in FP64.
+        // hundred terms: about 1e-6 relative; the forces only shape the
+        // proposal, whose ratio uses the same computed forces, and dE keeps
+        // its compensated sum) and the lanes' sums are added in FP64.
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
         float f6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         for (int jj = l; jj < nj; jj += G) {
