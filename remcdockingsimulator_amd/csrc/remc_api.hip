@@ -129,7 +129,8 @@ struct remc_ctx {
 
     // device buffers
     int *d_tc = nullptr, *d_body = nullptr, *d_bstart = nullptr, *d_bend = nullptr, *d_bcrow = nullptr;
-    float *d_q = nullptr, *d_sig = nullptr, *d_etab = nullptr;
+    float *d_q = nullptr, *d_sig = nullptr, *d_etab = nullptr, *d_sstab = nullptr;
+    int sigtab = 0;
     DevSys *d_sys = nullptr;
     DevRep *d_rep = nullptr;
     float *d_px = nullptr, *d_py = nullptr, *d_pz = nullptr, *d_rx = nullptr, *d_ry = nullptr, *d_rz = nullptr;
@@ -190,6 +191,7 @@ struct remc_ctx {
         p.tc = d_tc; p.q = d_q; p.sig = d_sig; p.body = d_body;
         p.bstart = d_bstart; p.bend = d_bend; p.bcrow = d_bcrow;
         p.sys = d_sys; p.rep = d_rep; p.etab = d_etab; p.ff = devff;
+        p.sstab = d_sstab; p.sigtab = sigtab;
         p.px = d_px; p.py = d_py; p.pz = d_pz; p.rx = d_rx; p.ry = d_ry; p.rz = d_rz;
         p.cx = d_cx; p.cy = d_cy; p.cz = d_cz; p.quat = d_quat;
         p.E = d_E; p.T = d_T; p.step = d_step; p.cnt = d_cnt;
@@ -457,6 +459,50 @@ int finalize(remc_ctx *c)
     }
 
     c->max_tiles = (c->max_beads + FULL_TILE - 1) / FULL_TILE;
+
+    // When every non-crowder bead's sigma is its residue type's (as the
+    // reference's amino-acid table makes it), the MC kernel reads each type
+    // pair's (0.5 (sigma_i + sigma_j))^2 / box^2 from a table per system: the
+    // FP32 operations the kernel would form from the two sigmas, in the same
+    // order, so the numbers are the same.
+    {
+        std::vector<float> sstab;
+        bool by_type = true;
+        for (const auto &hs : c->systems) {
+            float st[REMC_NTYPES];
+            bool seen[REMC_NTYPES] = {};
+            for (int m = 0; m < hs.n_bodies && by_type; ++m) {
+                if (hs.crowder[m]) continue;
+                for (int b = hs.body_start[m]; b < hs.body_start[m + 1]; ++b) {
+                    const int t = hs.type[b] & 0xff;
+                    if (!seen[t]) {
+                        seen[t] = true;
+                        st[t] = hs.sigma[b];
+                    } else if (!(st[t] == hs.sigma[b])) {
+                        by_type = false;
+                        break;
+                    }
+                }
+            }
+            if (!by_type) break;
+            const float ib = hs.box > 0.f ? 1.0f / hs.box : 0.f;
+            const float ib2 = ib * ib;
+            for (int i = 0; i < REMC_NTYPES; ++i)
+                for (int j = 0; j < REMC_NTYPES; ++j) {
+                    float v = 0.f;
+                    if (seen[i] && seen[j]) {
+                        const float h = 0.5f * st[i] + 0.5f * st[j];
+                        v = (h * h) * ib2;
+                    }
+                    sstab.push_back(v);
+                }
+        }
+        c->sigtab = by_type ? 1 : 0;
+        if (by_type) {
+            HIP_TRY(dalloc(&c->d_sstab, sstab.size()));
+            HIP_TRY(upload(c->d_sstab, sstab, c->stream));
+        }
+    }
 
     hipStream_t s = c->stream;
     HIP_TRY(dalloc(&c->d_tc, tc.size()));
@@ -979,7 +1025,7 @@ int remc_ctx_destroy(remc_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     collect_timing(c);
     destroy_graph(c);
-    void *ptrs[] = {c->d_tc, c->d_body, c->d_bstart, c->d_bend, c->d_bcrow, c->d_q, c->d_sig, c->d_etab,
+    void *ptrs[] = {c->d_tc, c->d_body, c->d_bstart, c->d_bend, c->d_bcrow, c->d_q, c->d_sig, c->d_etab, c->d_sstab,
                     c->d_sys, c->d_rep, c->d_px, c->d_py, c->d_pz, c->d_rx, c->d_ry, c->d_rz,
                     c->d_cx, c->d_cy, c->d_cz, c->d_quat, c->d_E, c->d_T, c->d_step, c->d_cnt,
                     c->d_slot, c->d_exoff, c->d_ladder, c->d_exround, c->d_exswaps, c->d_full_partials,

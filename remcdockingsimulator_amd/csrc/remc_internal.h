@@ -62,6 +62,8 @@ struct KParams {
     const DevSys *sys;
     const DevRep *rep;
     const float *etab;
+    const float *sstab; // per system: (0.5 (sigma_i + sigma_j))^2 / box^2 of type pairs (when sigtab)
+    int sigtab;         // 1: every non-crowder bead's sigma is its type's
     DevFF ff;
     // dynamic
     float *px, *py, *pz, *rx, *ry, *rz, *cx, *cy, *cz;

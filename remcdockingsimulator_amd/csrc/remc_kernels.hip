@@ -31,6 +31,7 @@
 // x^6 is formed by multiplication; exp(-r/xi) = exp2(r*k_dh) with the
 // hardware exp2; 1/r from the hardware reciprocal square root.
 #include <algorithm>
+#include <type_traits>
 
 #include "remc_internal.h"
 #include "remc_math.h"
@@ -369,6 +370,8 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     static_assert(WK <= 32 && WK <= REMC_SLOT_K, "window slots");
     constexpr int WS = SPEC ? WK * SPEC_MAX : 1;
     __shared__ float s_etab[REMC_NTYPES * REMC_NTYPES];
+    // (4 E_ij, and with p.sigtab the replica's system's sigma_ij^2 / box^2) per type pair
+    __shared__ float2 s_et2[REMC_NTYPES * REMC_NTYPES];
     __shared__ double s_red[NW];
     __shared__ double s_dE;
     __shared__ int s_abort;
@@ -416,7 +419,10 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     const DevSys S = p.sys[R.sys];
     const DevFF ff = p.ff;
 
-    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) s_etab[k] = p.etab[k];
+    for (int k = tid; k < REMC_NTYPES * REMC_NTYPES; k += BLOCK) {
+        s_etab[k] = p.etab[k];
+        s_et2[k] = make_float2(p.etab[k], p.sigtab ? p.sstab[(size_t)R.sys * (REMC_NTYPES * REMC_NTYPES) + k] : 0.0f);
+    }
     if (tid == 0) s_abort = 0;
 
     // the tracked energy and the counters live in LDS, updated by thread 0
@@ -1219,31 +1225,47 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                     hj[u] = 0.5f * sj[u];
                 }
                 const int t0 = tcj[0] & 0xff, t1 = tcj[1] & 0xff;
-                for (int k = k0; k < k1; ++k) {
-                    const float4 o = s_olds[k], n = s_news[k];
-                    const float *erow = s_etab + (s_tc[k] & 0xff) * REMC_NTYPES;
-                    const float e0 = erow[t0], e1 = erow[t1];
-                    const float h0 = n.w + hj[0], h1 = n.w + hj[1];
-                    const float ss0 = (h0 * h0) * ib2, ss1 = (h1 * h1) * ib2;
-                    const float ro0 = dist2_unit(xs[0], ys[0], zs[0], o), rn0 = dist2_unit(xs[0], ys[0], zs[0], n);
-                    const float ro1 = dist2_unit(xs[1], ys[1], zs[1], o), rn1 = dist2_unit(xs[1], ys[1], zs[1], n);
-                    float eo0, en0, eo1, en1;
-                    if (__any(o.w != 0.0f)) {
-                        const float qq0 = o.w * qj[0], qq1 = o.w * qj[1];
-                        eo0 = pair_lj_dh_k(ro0, e0, ss0, qq0, kdh);
-                        en0 = pair_lj_dh_k(rn0, e0, ss0, qq0, kdh);
-                        eo1 = pair_lj_dh_k(ro1, e1, ss1, qq1, kdh);
-                        en1 = pair_lj_dh_k(rn1, e1, ss1, qq1, kdh);
-                    } else {
-                        eo0 = pair_lj(ro0, e0, ss0);
-                        en0 = pair_lj(rn0, e0, ss0);
-                        eo1 = pair_lj(ro1, e1, ss1);
-                        en1 = pair_lj(rn1, e1, ss1);
+                // the pass over this lane's moved beads; with the type table
+                // (TAB) sigma_ij^2 / box^2 comes with E_ij in one LDS read
+                auto fast_pass = [&](auto tab) {
+                    constexpr bool TAB = decltype(tab)::value;
+                    for (int k = k0; k < k1; ++k) {
+                        const float4 o = s_olds[k], n = s_news[k];
+                        const float2 *erow = s_et2 + (s_tc[k] & 0xff) * REMC_NTYPES;
+                        const float2 e0 = erow[t0], e1 = erow[t1];
+                        float ss0, ss1;
+                        if (TAB) {
+                            ss0 = e0.y;
+                            ss1 = e1.y;
+                        } else {
+                            const float h0 = n.w + hj[0], h1 = n.w + hj[1];
+                            ss0 = (h0 * h0) * ib2;
+                            ss1 = (h1 * h1) * ib2;
+                        }
+                        const float ro0 = dist2_unit(xs[0], ys[0], zs[0], o), rn0 = dist2_unit(xs[0], ys[0], zs[0], n);
+                        const float ro1 = dist2_unit(xs[1], ys[1], zs[1], o), rn1 = dist2_unit(xs[1], ys[1], zs[1], n);
+                        float eo0, en0, eo1, en1;
+                        if (__any(o.w != 0.0f)) {
+                            const float qq0 = o.w * qj[0], qq1 = o.w * qj[1];
+                            eo0 = pair_lj_dh_k(ro0, e0.x, ss0, qq0, kdh);
+                            en0 = pair_lj_dh_k(rn0, e0.x, ss0, qq0, kdh);
+                            eo1 = pair_lj_dh_k(ro1, e1.x, ss1, qq1, kdh);
+                            en1 = pair_lj_dh_k(rn1, e1.x, ss1, qq1, kdh);
+                        } else {
+                            eo0 = pair_lj(ro0, e0.x, ss0);
+                            en0 = pair_lj(rn0, e0.x, ss0);
+                            eo1 = pair_lj(ro1, e1.x, ss1);
+                            en1 = pair_lj(rn1, e1.x, ss1);
+                        }
+                        // the two partners' differences are added (one rounding, as
+                        // each difference has), then to the compensated sum
+                        kahan_add(acc, comp, (en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
                     }
-                    // the two partners' differences are added (one rounding, as
-                    // each difference has), then to the compensated sum
-                    kahan_add(acc, comp, (en0 - eo0) + (on[1] ? en1 - eo1 : 0.0f));
-                }
+                };
+                if (p.sigtab)
+                    fast_pass(std::true_type{});
+                else
+                    fast_pass(std::false_type{});
                 continue;
             }
             if (__all(cr0 && (!on[1] || cr1))) {

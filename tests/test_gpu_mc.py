@@ -19,8 +19,8 @@ pytestmark = pytest.mark.gpu
 SEED = 0xC0FFEE
 
 
-def run_parity(oracle, cid, nsteps, block=None, replicas=None, coop=None, window=True):
-    wl = synth.config(cid)
+def run_parity(oracle, cid, nsteps, block=None, replicas=None, coop=None, window=True, wl=None):
+    wl = synth.config(cid) if wl is None else wl
     sa = wl.systems[0]
     lad = wl.temperatures[0]
     e = engine_for(wl, seed=SEED, block=block)
@@ -68,6 +68,22 @@ def test_mc_parity_config2(oracle):
 
 def test_mc_parity_config4(oracle):
     assert run_parity(oracle, 4, 300, replicas=range(0, 32, 4)) <= 1
+
+
+@pytest.mark.parametrize("mode", ["per_type", "per_bead"])
+def test_mc_sigma_per_type_and_per_bead(oracle, mode):
+    """Config 4 with bead sigmas that differ between residue types (the
+    large-move pass then reads sigma_ij^2 from the per-system type table) and
+    with sigmas that differ bead by bead within a type (the pass forms
+    sigma_ij^2 from the two beads' sigmas): both against the oracle."""
+    wl = synth.config(4)
+    sa = wl.systems[0]
+    if mode == "per_type":
+        sa.sigma = (5.0 + 0.25 * (sa.type % 7)).astype(np.float32)
+    else:
+        rng = np.random.default_rng(5)
+        sa.sigma = (6.0 + rng.uniform(-0.5, 0.5, len(sa.sigma))).astype(np.float32)
+    assert run_parity(oracle, 4, 300, replicas=[0, 13, 31], wl=wl) <= 1
 
 
 @pytest.mark.parametrize("block", [256, 512, 1024])
