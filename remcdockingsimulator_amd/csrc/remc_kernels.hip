@@ -944,6 +944,34 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             if (wave != 0 && nx_big) stage_move(tid - 64, BLOCK - 64, s_draw[nxo]);
             if (nx_big) pre_it = it + w;
 
+            // The commit's data (everything but the decisions) is read into
+            // registers by the upper half of the workgroup meanwhile: thread
+            // BLOCK/2 + t commits slot t / SPEC_MAX, bead t % SPEC_MAX.
+            const int ct = tid - BLOCK / 2;
+            const bool cthr = ct >= 0 && ct < WK * SPEC_MAX;
+            const int ck = cthr ? ct / SPEC_MAX : 0, ci = cthr ? ct % SPEC_MAX : 0;
+            bool cin = false, crot = false;
+            int cb = 0, cbk = 0;
+            float4 cnew = make_float4(0.f, 0.f, 0.f, 0.f), crel = cnew, ccen = cnew;
+            double cq[4] = {0.0, 0.0, 0.0, 0.0};
+            if (cthr && ck < w) {
+                const StepDraw &Dk = s_draw[off + ck];
+                cbk = Dk.body;
+                crot = Dk.rotate != 0;
+                cin = ci < Dk.nm;
+                cb = Dk.m0 + ci;
+                if (cin) {
+                    cnew = s_wnew[ct];
+                    if (crot) crel = s_wrel[ct];
+                    if (ci == 0) {
+                        if (crot)
+                            for (int t = 0; t < 4; ++t) cq[t] = s_wq[4 * ck + t];
+                        else
+                            ccen = s_wcen[ck];
+                    }
+                }
+            }
+
             // decisions in step order (wave 0); lane l carries slot l's dE
             if (wave == 0) {
                 // the slot's uniform and masks, read before the wait for the partials
@@ -1042,58 +1070,46 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
             // relative positions and the quaternion; translation: the centre),
             // and the earlier slot only what the later one leaves, so every
             // location has one writer.
-            if (tid < WK * SPEC_MAX) {
-                const int k = tid / SPEC_MAX, i = tid % SPEC_MAX;
-                if (k < weff && s_wdec[k]) {
-                    const StepDraw &Dk = s_draw[off + k];
-                    const int bk = Dk.body;
-                    const int m0k = Dk.m0, nmk = Dk.nm;
-                    const int nx = s_wnext[k];
-                    const bool later = nx >= 0;
-                    const bool later_rot = later && s_draw[off + nx].rotate;
-                    if (i < nmk) {
-                        const int b = m0k + i;
-                        if (!later) {
-                            const float4 n = s_wnew[tid];
-                            gx[b] = n.x;
-                            gy[b] = n.y;
-                            gz[b] = n.z;
-                            if (b >= clo && b < chi) {
-                                s_jx[b - clo] = n.x;
-                                s_jy[b - clo] = n.y;
-                                s_jz[b - clo] = n.z;
+            if (cthr && cin && ck < weff && s_wdec[ck]) {
+                const int nx = s_wnext[ck];
+                const bool later = nx >= 0;
+                const bool later_rot = later && s_draw[off + nx].rotate;
+                const int b = cb, bk = cbk;
+                if (!later) {
+                    gx[b] = cnew.x;
+                    gy[b] = cnew.y;
+                    gz[b] = cnew.z;
+                    if (b >= clo && b < chi) {
+                        s_jx[b - clo] = cnew.x;
+                        s_jy[b - clo] = cnew.y;
+                        s_jz[b - clo] = cnew.z;
+                    }
+                }
+                if (crot && !later_rot) {
+                    grx[b] = crel.x;
+                    gry[b] = crel.y;
+                    grz[b] = crel.z;
+                    if (cache_rel) {
+                        s_rx[b] = crel.x;
+                        s_ry[b] = crel.y;
+                        s_rz[b] = crel.z;
+                    }
+                }
+                if (ci == 0) {
+                    if (crot) {
+                        if (!later_rot)
+                            for (int t = 0; t < 4; ++t) {
+                                gq[4 * bk + t] = cq[t];
+                                if (cache_quat) s_q[4 * bk + t] = cq[t];
                             }
-                        }
-                        if (Dk.rotate && !later_rot) {
-                            const float4 nr = s_wrel[tid];
-                            grx[b] = nr.x;
-                            gry[b] = nr.y;
-                            grz[b] = nr.z;
-                            if (cache_rel) {
-                                s_rx[b] = nr.x;
-                                s_ry[b] = nr.y;
-                                s_rz[b] = nr.z;
-                            }
-                        }
-                        if (i == 0) {
-                            if (Dk.rotate) {
-                                if (!later_rot)
-                                    for (int t = 0; t < 4; ++t) {
-                                        const double qn = s_wq[4 * k + t];
-                                        gq[4 * bk + t] = qn;
-                                        if (cache_quat) s_q[4 * bk + t] = qn;
-                                    }
-                            } else if (!later || later_rot) {
-                                const float4 nc = s_wcen[k];
-                                gcx[bk] = nc.x;
-                                gcy[bk] = nc.y;
-                                gcz[bk] = nc.z;
-                                if (cache_bodies) {
-                                    s_cx[bk] = nc.x;
-                                    s_cy[bk] = nc.y;
-                                    s_cz[bk] = nc.z;
-                                }
-                            }
+                    } else if (!later || later_rot) {
+                        gcx[bk] = ccen.x;
+                        gcy[bk] = ccen.y;
+                        gcz[bk] = ccen.z;
+                        if (cache_bodies) {
+                            s_cx[bk] = ccen.x;
+                            s_cy[bk] = ccen.y;
+                            s_cz[bk] = ccen.z;
                         }
                     }
                 }
