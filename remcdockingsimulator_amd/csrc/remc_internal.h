@@ -146,6 +146,7 @@ __host__ __device__ inline int full_units(int tiles) { return tiles * (tiles / 2
 // largest number of partner sub-ranges a full-energy unit is cut into
 constexpr int FULL_SPLIT_MAX = 4;
 constexpr int SPEC_MAX = 16;     // largest body (beads) a speculative window slot takes
+static_assert(SPEC_MAX > 1, "a window slot's thread 1 forms its rotation");
 constexpr int REMC_MAX_COOP = 8; // workgroups per replica in the cooperative MC kernel
 constexpr int REMC_SLOT_K = 64;  // partial sums per member and exchange (MC window slots, force-biased window values)
 // two banks of REMC_SLOT_K x REMC_MAX_COOP tagged partial sums per replica

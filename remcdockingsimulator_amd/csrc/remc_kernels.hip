@@ -729,23 +729,22 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
                         s_wstat[k] = nmk == 1 && ox == nrx + ncx && oy == nry + ncy && oz == nrz + ncz ? 1 : 0;
                     s_wrel[st] = make_float4(nrx, nry, nrz, qb);
                     s_wtc[st] = inc ? lds_ld(s_jtc + (b - clo)) : tcs[b];
-                    if (i == 0) {
-                        if (Dk.rotate) {
-                            double qo[4], qn[4];
-                            if (cache_quat)
-                                for (int t = 0; t < 4; ++t) qo[t] = lds_ld(s_q + 4 * bk + t);
-                            else
-                                for (int t = 0; t < 4; ++t) qo[t] = gq[4 * bk + t];
-                            if (chained && sd[off_ + pj].rotate) {
-                                remc_qmul(sd[off_ + pj].q, qo, qn);
-                                for (int t = 0; t < 4; ++t) qo[t] = qn[t];
-                            }
-                            remc_qmul(Dk.q, qo, qn);
-                            for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
-                        } else {
-                            s_wcen[k] = make_float4(ncx, ncy, ncz, 0.0f);
-                        }
+                    if (i == 0 && !Dk.rotate) s_wcen[k] = make_float4(ncx, ncy, ncz, 0.0f);
+                }
+                // a rotated slot's new rotation, formed by the slot's thread 1
+                // beside bead 0's positions (SPEC_MAX > 1)
+                if (i == 1 && Dk.rotate) {
+                    double qo[4], qn[4];
+                    if (cache_quat)
+                        for (int t = 0; t < 4; ++t) qo[t] = lds_ld(s_q + 4 * bk + t);
+                    else
+                        for (int t = 0; t < 4; ++t) qo[t] = gq[4 * bk + t];
+                    if (chained && sd[off_ + pj].rotate) {
+                        remc_qmul(sd[off_ + pj].q, qo, qn);
+                        for (int t = 0; t < 4; ++t) qo[t] = qn[t];
                     }
+                    remc_qmul(Dk.q, qo, qn);
+                    for (int t = 0; t < 4; ++t) s_wq[4 * k + t] = qn[t];
                 }
             }
         };
