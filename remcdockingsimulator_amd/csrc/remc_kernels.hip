@@ -202,39 +202,11 @@ __device__ __forceinline__ int div_floor_small(int a, int b)
 
 __device__ __forceinline__ int div_ceil_small(int a, int b) { return div_floor_small(a + b - 1, b); }
 
-// Work split of one step's (moved bead x partner bead) pairs over a
-// workgroup: groups of G lanes, lane l of group g takes partners l, l+G, ...
-// (t = ceil(nj/G) of them, held in registers two at a time) against moved
-// beads [g*kper, (g+1)*kper).  A moved bead's staging is read from LDS once
-// per pair of partners, and that LDS traffic weighs as much as the pair
-// arithmetic, so a split costs kper * (t + 2 ceil(t/2)); of the splits with
-// t = ceil(nj/BLOCK) .. +3 partner iterations per lane the cheapest is taken.
-template <int BLOCK>
-__device__ __forceinline__ void pick_split(int nm, int nj, int &G, int &kper)
-{
-    G = 1;
-    kper = nm;
-    if (nj <= 0) return;
-    const int t1 = div_ceil_small(nj, BLOCK);
-    int best = 0x7fffffff;
-#pragma unroll
-    for (int dt = 0; dt < 4; ++dt) {
-        const int t = t1 + dt;
-        const int Gt = div_ceil_small(nj, t);
-        const int groups = div_floor_small(BLOCK, Gt);
-        const int kp = div_ceil_small(nm, groups);
-        const int cost = kp * (t + 2 * ((t + 1) >> 1));
-        if (cost < best) {
-            best = cost;
-            G = Gt;
-            kper = kp;
-        }
-    }
-}
-
-// The split of the MC kernel's pair loop, which holds two partners per lane
-// per pass and runs all four terms of a pass whether or not the second
-// partner exists.  Lane i of each wave prices g = i + 1 groups of
+// The split of a large move's (moved bead x partner bead) pairs over a
+// workgroup, for pair loops (MC and force-biased) that hold two partners per
+// lane per pass and run all terms of a pass whether or not the second partner
+// exists: lane l of group g takes partners l, l + G, ... against moved beads
+// [g kper, (g + 1) kper).  Lane i of each wave prices g = i + 1 groups of
 // G = BLOCK / g lanes: ceil(ceil(nj / G) / 2) passes, each over
 // kper = ceil(nm / g) moved beads plus a quarter for the pass's partner loads,
 // 5% more where group boundaries fall inside waves (a wave then spans several
@@ -2139,33 +2111,58 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         const int nj = je - js;
 
         // pass A: force and torque at the old position; the (moved bead x
-        // partner bead) pairs split over the workgroup as in the MC kernel
+        // partner bead) pairs split over the workgroup as in the MC kernel,
+        // two partner beads per lane per pass (jj0 and jj0 + G)
         int G, kper;
-        pick_split<BLOCK>(nm, nj, G, kper);
+        pick_split_pairs<BLOCK>(nm, nj, lane, G, kper);
+        G = __builtin_amdgcn_readfirstlane(G);
+        kper = __builtin_amdgcn_readfirstlane(kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
+        // partner jj's coordinates, charge, sigma and type (jj < nj)
+        auto load_partner = [&](const int jj, float &xj, float &yj, float &zj, float &qj, float &sj, int &tcj) {
+            const int j = js + jj + (js + jj >= ovs ? ov : 0);
+            xj = cached ? s_jx[j] : gx[j];
+            yj = cached ? s_jy[j] : gy[j];
+            zj = cached ? s_jz[j] : gz[j];
+            qj = cached ? s_jq[j] : qs[j];
+            sj = cached ? s_js[j] : sgs[j];
+            tcj = cached ? s_jtc[j] : tcs[j];
+        };
         // Each lane sums its pairs' force and torque in FP32 (a few to a few
         // hundred terms: about 1e-6 relative; the forces only shape the
         // proposal, whose ratio uses the same computed forces, and dE keeps
         // its compensated sum) and the lanes' sums are added in FP64.
         double v[7] = {0, 0, 0, 0, 0, 0, 0};
         float f6[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-        for (int jj = l; jj < nj; jj += G) {
-            const int j = js + jj + (js + jj >= ovs ? ov : 0);
-            const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
-            const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
-            const int tcj = cached ? s_jtc[j] : tcs[j];
-            const bool crow = crow_m || (tcj >> 8) != 0;
+        for (int jj0 = l; jj0 < nj; jj0 += 2 * G) {
+            const bool on1 = jj0 + G < nj;
+            float xj[2], yj[2], zj[2], qj[2], sj[2];
+            int tcj[2];
+            bool crow[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                load_partner(u == 0 || !on1 ? jj0 : jj0 + G, xj[u], yj[u], zj[u], qj[u], sj[u], tcj[u]);
+                crow[u] = crow_m || (tcj[u] >> 8) != 0;
+            }
             for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k];
-                const float dx = min_image(o.x - xj, S.box, S.inv_box);
-                const float dy = min_image(o.y - yj, S.box, S.inv_box);
-                const float dz = min_image(o.z - zj, S.box, S.inv_box);
-                const float sij = 0.5f * (orl.w + sj);
-                float e, g2;
-                pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow,
-                        s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)], sij * sij, o.w * qj, inv_xi, ff, e, g2);
-                const float fx = -(g2 * dx), fy = -(g2 * dy), fz = -(g2 * dz);
+                const int trow = (s_tc[k] & 0xff) * REMC_NTYPES;
+                float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const float dx = min_image(o.x - xj[u], S.box, S.inv_box);
+                    const float dy = min_image(o.y - yj[u], S.box, S.inv_box);
+                    const float dz = min_image(o.z - zj[u], S.box, S.inv_box);
+                    const float sij = 0.5f * (orl.w + sj[u]);
+                    float e, g2;
+                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow[u],
+                            s_etab[trow + (tcj[u] & 0xff)], sij * sij, o.w * qj[u], inv_xi, ff, e, g2);
+                    if (u == 1 && !on1) g2 = 0.0f;
+                    fx -= g2 * dx;
+                    fy -= g2 * dy;
+                    fz -= g2 * dz;
+                }
                 f6[0] += fx;
                 f6[1] += fy;
                 f6[2] += fz;
@@ -2229,32 +2226,48 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         // pass B: dE and the force and torque at the new position (same split)
         float acc = 0.0f, comp = 0.0f;
         for (int k = 0; k < 6; ++k) f6[k] = 0.f;
-        for (int jj = l; jj < nj; jj += G) {
-            const int j = js + jj + (js + jj >= ovs ? ov : 0);
-            const float xj = cached ? s_jx[j] : gx[j], yj = cached ? s_jy[j] : gy[j], zj = cached ? s_jz[j] : gz[j];
-            const float qj = cached ? s_jq[j] : qs[j], sj = cached ? s_js[j] : sgs[j];
-            const int tcj = cached ? s_jtc[j] : tcs[j];
-            const bool crow = crow_m || (tcj >> 8) != 0;
+        for (int jj0 = l; jj0 < nj; jj0 += 2 * G) {
+            const bool on1 = jj0 + G < nj;
+            float xj[2], yj[2], zj[2], qj[2], sj[2];
+            int tcj[2];
+            bool crow[2];
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                load_partner(u == 0 || !on1 ? jj0 : jj0 + G, xj[u], yj[u], zj[u], qj[u], sj[u], tcj[u]);
+                crow[u] = crow_m || (tcj[u] >> 8) != 0;
+            }
             for (int k = k0; k < k1; ++k) {
                 const float4 o = s_old[k], orl = s_orel[k], n = s_new[k], nrl = s_nrel[k];
-                const float eij = s_etab[(s_tc[k] & 0xff) * REMC_NTYPES + (tcj & 0xff)];
-                const float sij = 0.5f * (orl.w + sj);
-                const float qq = o.w * qj;
-                float eo, en, go, g2;
-                {
-                    const float dx = min_image(o.x - xj, S.box, S.inv_box);
-                    const float dy = min_image(o.y - yj, S.box, S.inv_box);
-                    const float dz = min_image(o.z - zj, S.box, S.inv_box);
-                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi,
-                            ff, eo, go);
+                const int trow = (s_tc[k] & 0xff) * REMC_NTYPES;
+                float d = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const float eij = s_etab[trow + (tcj[u] & 0xff)];
+                    const float sij = 0.5f * (orl.w + sj[u]);
+                    const float qq = o.w * qj[u];
+                    float eo, en, go, g2;
+                    {
+                        const float dx = min_image(o.x - xj[u], S.box, S.inv_box);
+                        const float dy = min_image(o.y - yj[u], S.box, S.inv_box);
+                        const float dz = min_image(o.z - zj[u], S.box, S.inv_box);
+                        pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow[u], eij, sij * sij, qq,
+                                inv_xi, ff, eo, go);
+                    }
+                    const float dx = min_image(n.x - xj[u], S.box, S.inv_box);
+                    const float dy = min_image(n.y - yj[u], S.box, S.inv_box);
+                    const float dz = min_image(n.z - zj[u], S.box, S.inv_box);
+                    pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow[u], eij, sij * sij, qq, inv_xi,
+                            ff, en, g2);
+                    const bool use = u == 0 || on1;
+                    // the two partners' differences are added (one rounding, as
+                    // each difference has), then to the compensated sum
+                    d += use ? en - eo : 0.0f;
+                    if (!use) g2 = 0.0f;
+                    fx -= g2 * dx;
+                    fy -= g2 * dy;
+                    fz -= g2 * dz;
                 }
-                const float dx = min_image(n.x - xj, S.box, S.inv_box);
-                const float dy = min_image(n.y - yj, S.box, S.inv_box);
-                const float dz = min_image(n.z - zj, S.box, S.inv_box);
-                pair_eg(__builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, dz * dz)), crow, eij, sij * sij, qq, inv_xi, ff,
-                        en, g2);
-                kahan_add(acc, comp, en - eo);
-                const float fx = -(g2 * dx), fy = -(g2 * dy), fz = -(g2 * dz);
+                kahan_add(acc, comp, d);
                 f6[0] += fx;
                 f6[1] += fy;
                 f6[2] += fz;
