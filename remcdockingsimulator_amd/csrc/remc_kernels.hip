@@ -1520,22 +1520,28 @@ static hipError_t launch_mc_t(const KParams &p, int grid, int nsteps, int coop, 
 __device__ __forceinline__ void pair_eg(float r2, bool crow, float eij, float sij2, float qq, float inv_xi,
                                         const DevFF &ff, float &e, float &g)
 {
+    // The energy and its derivative share the reciprocal (root), x6 and the
+    // Debye-Hueckel exponential.  For r2 >= 1e-36 the energy is formed by the
+    // operations of pair_crowder / pair_lj_dh<false> on the same inputs, so it
+    // is the same number; below (coincident beads) both give the 1e30 cap.
+    const float r2c = fmaxf(r2, 1e-36f);
     if (crow) {
-        e = pair_crowder(r2, ff);
-        const float rc = __builtin_amdgcn_rcpf(fmaxf(r2, 1e-36f));
+        const float rc = __builtin_amdgcn_rcpf(r2c);
+        const float rc3 = rc * rc * rc;
+        e = r2 < ff.crow_cut2 ? fminf(ff.crow_k * rc3, 1e30f) : 0.0f;
         g = fmaxf(-6.0f * e * rc, -1e30f); // d(C r^-6)/dr / r = -6 E / r^2
         return;
     }
-    e = pair_lj_dh<false>(r2, eij, sij2, qq, ff);
-    r2 = fmaxf(r2, 1e-36f);
-    const float rinv = __builtin_amdgcn_rsqf(r2);
-    const float r = r2 * rinv;
+    const float rinv = __builtin_amdgcn_rsqf(r2c);
     const float s2 = sij2 * (rinv * rinv);
     const float x6 = s2 * s2 * s2;
-    const float dbase = (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv); // d[x6 (x6 - 1)]/dr (eij holds 4 E)
+    const float base = __builtin_fmaf(x6, x6, -x6);
     const bool flip = (eij > 0.0f) & (x6 > 0.5f);
-    const float dlj = flip ? eij * dbase : -eij * dbase;
+    const float r = r2c * rinv;
     const float dh = qq * __builtin_amdgcn_exp2f(r * ff.k_dh) * rinv;
+    e = fminf(eij * (flip ? base + 0.5f : -base) + dh, 1e30f);
+    const float dbase = (2.0f * x6 - 1.0f) * (-6.0f * x6 * rinv); // d[x6 (x6 - 1)]/dr (eij holds 4 E)
+    const float dlj = flip ? eij * dbase : -eij * dbase;
     const float ddh = -dh * (inv_xi + rinv);
     g = fmaxf(fminf((dlj + ddh) * rinv, 1e30f), -1e30f);
 }
