@@ -202,6 +202,21 @@ __device__ __forceinline__ int div_floor_small(int a, int b)
 
 __device__ __forceinline__ int div_ceil_small(int a, int b) { return div_floor_small(a + b - 1, b); }
 
+// floor(c n / d) for 0 <= c <= d and n >= 0 in 32-bit arithmetic (the bound
+// of member c's share of n partners over d members): with n = q d + r it is
+// c q + floor(c r / d), where c q <= n and c r < d^2
+__device__ __forceinline__ int share_bound(int c, int n, int d)
+{
+    const unsigned q = (unsigned)n / (unsigned)d, r = (unsigned)n - q * (unsigned)d;
+    return (int)((unsigned)c * q + ((unsigned)c * r) / (unsigned)d);
+}
+
+// The split choice (G, kper) of the last two move shapes (moved beads,
+// partner beads): a move of either shape takes it without the search.
+struct SplitMemo {
+    int nm[2] = {-1, -1}, nj[2] = {-1, -1}, G[2] = {1, 1}, kper[2] = {1, 1};
+};
+
 // The split of a large move's (moved bead x partner bead) pairs over a
 // workgroup, for pair loops (MC and force-biased) that hold two partners per
 // lane per pass and run all terms of a pass whether or not the second partner
@@ -234,6 +249,31 @@ __device__ __forceinline__ void pick_split_pairs(int nm, int nj, int lane, int &
     const int gb = (int)(key & 63u) + 1;
     G = div_floor_small(BLOCK, gb);
     kper = div_ceil_small(nm, gb);
+}
+
+// pick_split_pairs through the memo m (nm and nj uniform over the workgroup)
+template <int BLOCK>
+__device__ __forceinline__ void pick_split_memo(SplitMemo &m, int nm, int nj, int lane, int &G, int &kper)
+{
+    if (nm == m.nm[0] && nj == m.nj[0]) {
+        G = m.G[0];
+        kper = m.kper[0];
+    } else if (nm == m.nm[1] && nj == m.nj[1]) {
+        G = m.G[1];
+        kper = m.kper[1];
+    } else {
+        pick_split_pairs<BLOCK>(nm, nj, lane, G, kper);
+        G = __builtin_amdgcn_readfirstlane(G);
+        kper = __builtin_amdgcn_readfirstlane(kper);
+        m.nm[1] = m.nm[0];
+        m.nj[1] = m.nj[0];
+        m.G[1] = m.G[0];
+        m.kper[1] = m.kper[0];
+        m.nm[0] = nm;
+        m.nj[0] = nj;
+        m.G[0] = G;
+        m.kper[0] = kper;
+    }
 }
 
 // Draws of one MC step and the move parameters that depend on them only
@@ -465,6 +505,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
     int nb0 = 0;
     int pre_w = 0;       // slots of the next window, staged ahead (0: none)
     int pre_it = -1;     // the large move staged ahead (launch step index; -1: none)
+    SplitMemo smemo;     // the split of the latest large-move shapes
     int it = 0;
     while (it < nsteps) {
         // The thread index is re-read opaquely in every step, so that the
@@ -1138,23 +1179,23 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
 
         // Moves with few pair terms are evaluated whole by every member (no
         // exchange); larger ones are split over the members' partner slices.
-        const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
+        // (the body's bead range is uniform: the integer work below is scalar)
+        const int m0u = __builtin_amdgcn_readfirstlane(m0), nmu = __builtin_amdgcn_readfirstlane(nm);
+        const bool split = coop > 1 && (long long)nmu * (long long)(N - nmu) >= (long long)p.coop_min_pairs;
         // this member's share of dE = sum_{i in body} sum_{j in share} e(new_i,j) - e(old_i,j),
         // split into groups of moved beads x G lanes over the share's partner
         // beads: an even part of the N - nm partners (index v, bead v, or
         // v + nm past the moved body) when the member can read every bead,
         // else its cached slice without the moved body's overlap [ovs, ovs + ov)
         const bool even = split && !(CACHE && p.cache_slice);
-        const int np = N - nm;
-        const int js = !split ? 0 : even ? (int)(((long long)c * np) / coop) : cjs;
-        const int je = !split ? N : even ? (int)(((long long)(c + 1) * np) / coop) : cje;
-        const int ovs = even ? m0 : max(js, m0);
-        const int ov = even ? nm : max(0, min(je, m1) - ovs);
+        const int np = N - nmu;
+        const int js = !split ? 0 : even ? share_bound(c, np, coop) : cjs;
+        const int je = !split ? N : even ? share_bound(c + 1, np, coop) : cje;
+        const int ovs = even ? m0u : max(js, m0u);
+        const int ov = even ? nmu : max(0, min(je, m0u + nmu) - ovs);
         const int nj = even ? je - js : (je - js) - ov;
         int G, kper;
-        pick_split_pairs<BLOCK>(nm, nj, lane, G, kper);
-        G = __builtin_amdgcn_readfirstlane(G);
-        kper = __builtin_amdgcn_readfirstlane(kper);
+        pick_split_memo<BLOCK>(smemo, nmu, nj, lane, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         // compensated (Kahan) FP32 sum of the differences en - eo per thread
@@ -1709,6 +1750,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
     };
 
     int fb0 = 0, fhave = 0; // s_fbd holds the draws of steps [fb0, fb0 + fhave) of this launch
+    SplitMemo smemo;        // the split of the latest large-move shapes
     for (int it = 0; it < nsteps; ++it, ++step) {
         // the thread index re-read opaquely in every step, as in k_mc: the
         // addresses derived from it are formed where used, not held in registers
@@ -2107,12 +2149,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         // A large move is split over the members' shares of the partners, the
         // partial forces, torques and dE then exchanged; a small one is
         // evaluated whole by every member, with the same result in each.
-        const bool split = coop > 1 && (long long)nm * (long long)(N - nm) >= (long long)p.coop_min_pairs;
+        const int nmu = __builtin_amdgcn_readfirstlane(nm); // uniform: scalar integer work
+        const bool split = coop > 1 && (long long)nmu * (long long)(N - nmu) >= (long long)p.coop_min_pairs;
         // a member's share: an even part of the N - nm partners (index v, bead
         // v, or v + nm past the moved body)
-        const int np = N - nm;
-        const int js = split ? (int)(((long long)c * np) / coop) : 0;
-        const int je = split ? (int)(((long long)(c + 1) * np) / coop) : np;
+        const int np = N - nmu;
+        const int js = split ? share_bound(c, np, coop) : 0;
+        const int je = split ? share_bound(c + 1, np, coop) : np;
         const int ovs = m0, ov = nm;
         const int nj = je - js;
 
@@ -2120,9 +2163,7 @@ __global__ __launch_bounds__(BLOCK) void k_mc_fb(KParams p, int nsteps, int coop
         // partner bead) pairs split over the workgroup as in the MC kernel,
         // two partner beads per lane per pass (jj0 and jj0 + G)
         int G, kper;
-        pick_split_pairs<BLOCK>(nm, nj, lane, G, kper);
-        G = __builtin_amdgcn_readfirstlane(G);
-        kper = __builtin_amdgcn_readfirstlane(kper);
+        pick_split_memo<BLOCK>(smemo, nmu, nj, lane, G, kper);
         const int g = div_floor_small(tid, G), l = tid - g * G;
         const int k0 = min(nm, g * kper), k1 = min(nm, k0 + kper);
         // partner jj's coordinates, charge, sigma and type (jj < nj)
