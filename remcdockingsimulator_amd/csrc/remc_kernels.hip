@@ -613,18 +613,15 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         // box-scaled copies and s_tc, by threads t0, t0 + stride, ... over its
         // beads.  It reads only the moved body.
         auto stage_move = [&](const int t0, const int stride, const StepDraw &Dm) {
-            const int bd = Dm.body;
-            int mb0, mb1;
+            // the bead range comes with the draw: the centre and the beads'
+            // reads wait on the draw only
+            const int bd = Dm.body, mb0 = Dm.m0, mb1 = Dm.m0 + Dm.nm;
             float bcx, bcy, bcz;
             if (cache_bodies) {
-                mb0 = lds_ld(s_bs + bd);
-                mb1 = lds_ld(s_be + bd);
                 bcx = lds_ld(s_cx + bd);
                 bcy = lds_ld(s_cy + bd);
                 bcz = lds_ld(s_cz + bd);
             } else {
-                mb0 = p.bstart[S.body_off + bd];
-                mb1 = p.bend[S.body_off + bd];
                 bcx = gcx[bd];
                 bcy = gcy[bd];
                 bcz = gcz[bd];
@@ -1153,18 +1150,13 @@ __global__ __launch_bounds__(BLOCK) void k_mc(KParams p, int nsteps, int coop)
         pre_it = -1;
         const int body = D.body;
         const bool rotate = D.rotate != 0;
-        int m0, m1;
+        const int m0 = D.m0, m1 = D.m0 + D.nm; // the bead range formed with the draw
         float cx, cy, cz;
         if (cache_bodies) {
-            m0 = lds_ld(s_bs + body);
-            m1 = lds_ld(s_be + body);
             cx = lds_ld(s_cx + body);
             cy = lds_ld(s_cy + body);
             cz = lds_ld(s_cz + body);
         } else {
-            const int gb = S.body_off + body;
-            m0 = p.bstart[gb];
-            m1 = p.bend[gb];
             cx = gcx[body];
             cy = gcy[body];
             cz = gcz[body];
