@@ -150,6 +150,7 @@ struct remc_ctx {
     int *d_err = nullptr;
     double *d_adj = nullptr; // adjoint scratch: bead gradients, tile partials, body forces, parameter gradients
     size_t adj_cap = 0;      // doubles held by d_adj
+    int adj_target = 0;      // all-replica adjoint's workgroup target (0: tune on first use)
     void *d_ana = nullptr;   // analysis scratch (reference conformation, partials, histogram)
     size_t ana_bytes = 0;
     DevFF devff{};
@@ -607,6 +608,7 @@ int finalize(remc_ctx *c)
     KParams p = c->params();
     HIP_TRY(launch_energy_full(p, R, c->max_tiles, c->d_full_partials, nullptr, 0, 1, 0, s));
     HIP_TRY(hipStreamSynchronize(s));
+    c->adj_target = 0; // the all-replica adjoint is tuned on its first call
     return autotune_energy(c);
 }
 
@@ -958,10 +960,10 @@ int ensure_mc_config(remc_ctx *c)
 // entries of the replica arrays from bead_base, their bodies n_bodies_out
 // entries from body_base.  Outputs: 6 doubles per body in replica order, 3
 // parameter derivatives per replica.
-int adjoint_range(remc_ctx *c, int first, int n, int max_beads, int max_bodies, int bead_base, size_t bead_stride,
-                  int body_base, int n_bodies_out, double *force_torque, double *dparams)
+// adjoint scratch of plan a for n replicas and n_bodies_out output bodies,
+// grown as needed
+int adjoint_scratch(remc_ctx *c, const AdjPlan &a, int n, int n_bodies_out)
 {
-    const AdjPlan a = adjoint_plan(n, max_beads, bead_stride);
     const size_t need = a.part_doubles + a.param_doubles + 6 * (size_t)n_bodies_out + 3 * (size_t)n;
     if (need > c->adj_cap) {
         if (c->d_adj) HIP_TRY(hipFree(c->d_adj));
@@ -970,10 +972,87 @@ int adjoint_range(remc_ctx *c, int first, int n, int max_beads, int max_bodies, 
         HIP_TRY(hipMalloc(&c->d_adj, need * sizeof(double)));
         c->adj_cap = need;
     }
-    double *ft = c->d_adj + a.part_doubles + a.param_doubles, *dp = ft + 6 * (size_t)n_bodies_out;
+    return REMC_OK;
+}
+
+std::vector<unsigned char> adjoint_ff(const remc_ctx *c)
+{
     std::vector<unsigned char> af(adjoint_ff_size());
     adjoint_ff_fill(af.data(), c->ff.table, c->ff.lambda, c->ff.e0, c->ff.xi, c->ff.lj_cf * c->ff.kbt_cf,
                     c->ff.dh_cf * c->ff.kbt_cf);
+    return af;
+}
+
+// Workgroup target of the all-replica adjoint, from which its partner split
+// follows: timed once per process and (replicas, beads), so every context of
+// that size takes the same split and the same bits.  Each distinct split is
+// run on the live conformations (read only), best of three; the size rule's
+// target is kept unless another is at least 3% faster.
+std::map<std::pair<int, int>, int> g_adj_cache; // under g_split_mu
+
+int autotune_adjoint(remc_ctx *c)
+{
+    const int R = (int)c->reps.size();
+    const std::pair<int, int> key(R, c->max_beads);
+    {
+        std::lock_guard<std::mutex> lock(g_split_mu);
+        auto it = g_adj_cache.find(key);
+        if (it != g_adj_cache.end()) {
+            c->adj_target = it->second;
+            return REMC_OK;
+        }
+    }
+    const size_t stride = (size_t)c->total_rep_beads;
+    const int cand[4] = {2048, 1024, 4096, 8192}; // the size rule's first
+    float best_ms[4];
+    int split[4];
+    EventPair ev;
+    HIP_TRY(ev.create());
+    const KParams p = c->params();
+    const std::vector<unsigned char> af = adjoint_ff(c);
+    for (int k = 0; k < 4; ++k) {
+        const AdjPlan a = adjoint_plan(R, c->max_beads, stride, cand[k]);
+        split[k] = a.split;
+        best_ms[k] = 1e30f;
+        int same = -1;
+        for (int q = 0; q < k; ++q)
+            if (split[q] == a.split) same = q;
+        if (same >= 0) continue; // the same launch as an earlier candidate
+        int rc = adjoint_scratch(c, a, R, c->total_rep_bodies);
+        if (rc != REMC_OK) return rc;
+        double *ft = c->d_adj + a.part_doubles + a.param_doubles, *dp = ft + 6 * (size_t)c->total_rep_bodies;
+        HIP_TRY(launch_adjoint(p, af.data(), a, 0, R, c->max_bodies, 0, stride, 0, c->d_adj, ft, dp, c->stream));
+        for (int rep = 0; rep < 3; ++rep) {
+            HIP_TRY(hipEventRecord(ev.a, c->stream));
+            HIP_TRY(launch_adjoint(p, af.data(), a, 0, R, c->max_bodies, 0, stride, 0, c->d_adj, ft, dp, c->stream));
+            HIP_TRY(hipEventRecord(ev.b, c->stream));
+            HIP_TRY(hipEventSynchronize(ev.b));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, ev.a, ev.b));
+            best_ms[k] = std::min(best_ms[k], ms);
+        }
+    }
+    int pick = cand[0];
+    float t_pick = best_ms[0];
+    for (int k = 1; k < 4; ++k)
+        if (best_ms[k] < 0.97f * best_ms[0] && best_ms[k] < t_pick) {
+            pick = cand[k];
+            t_pick = best_ms[k];
+        }
+    std::lock_guard<std::mutex> lock(g_split_mu);
+    auto it = g_adj_cache.emplace(key, pick).first; // a concurrent tuner's entry wins
+    c->adj_target = it->second;
+    return REMC_OK;
+}
+
+int adjoint_range(remc_ctx *c, int first, int n, int max_beads, int max_bodies, int bead_base, size_t bead_stride,
+                  int body_base, int n_bodies_out, double *force_torque, double *dparams, int target_wg = 0)
+{
+    const AdjPlan a = adjoint_plan(n, max_beads, bead_stride, target_wg);
+    int rc = adjoint_scratch(c, a, n, n_bodies_out);
+    if (rc != REMC_OK) return rc;
+    double *ft = c->d_adj + a.part_doubles + a.param_doubles, *dp = ft + 6 * (size_t)n_bodies_out;
+    const std::vector<unsigned char> af = adjoint_ff(c);
     {
         TimedLaunch t(c, 4);
         HIP_TRY(launch_adjoint(c->params(), af.data(), a, first, n, max_bodies, bead_base, bead_stride, body_base,
@@ -1863,8 +1942,12 @@ int remc_adjoint_all(remc_ctx *c, double *force_torque, double *dparams)
     CHECK_CTX(c);
     FINALIZE(c);
     if (!force_torque && !dparams) return fail(REMC_ERR_ARG, "null outputs");
+    if (c->adj_target == 0) {
+        const int rc = autotune_adjoint(c);
+        if (rc != REMC_OK) return rc;
+    }
     return adjoint_range(c, 0, (int)c->reps.size(), c->max_beads, c->max_bodies, 0, (size_t)c->total_rep_beads, 0,
-                         c->total_rep_bodies, force_torque, dparams);
+                         c->total_rep_bodies, force_torque, dparams, c->adj_target);
 }
 
 int remc_timing_enable(remc_ctx *c, int enable)

@@ -126,7 +126,9 @@ struct AdjPlan {
     int tiles, split, sub_len;
     size_t part_doubles, param_doubles;
 };
-AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride);
+// target_wg: the workgroups a launch aims at (0: the size rule's 2048);
+// sub-ranges are cut only while the launch holds fewer
+AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride, int target_wg = 0);
 hipError_t launch_adjoint(const KParams &p, const void *adj_ff, const AdjPlan &a, int rep_first, int n_rep,
                           int max_bodies, int bead_base, size_t bead_stride, int body_base, double *scratch,
                           double *force_torque, double *dparams, hipStream_t s);

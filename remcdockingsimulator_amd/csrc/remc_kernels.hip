@@ -3009,12 +3009,13 @@ __global__ __launch_bounds__(256) void k_adjoint_bodies(KParams p, int rep_first
     }
 }
 
-AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride)
+AdjPlan adjoint_plan(int n_rep, int max_beads, size_t bead_stride, int target_wg)
 {
     AdjPlan a{};
     a.tiles = (max_beads + ADJ_ROWS - 1) / ADJ_ROWS;
+    const long long target = target_wg > 0 ? target_wg : ADJ_TARGET_WG;
     const long long units = (long long)n_rep * a.tiles;
-    const long long want = units >= ADJ_TARGET_WG ? 1 : (ADJ_TARGET_WG + units - 1) / units;
+    const long long want = units >= target ? 1 : (target + units - 1) / units;
     const long long per = (max_beads + want - 1) / want;
     a.sub_len = (int)std::max<long long>(ADJ_ROWS, (per + ADJ_ROWS - 1) / ADJ_ROWS * ADJ_ROWS);
     a.split = (max_beads + a.sub_len - 1) / a.sub_len;
